@@ -1,0 +1,153 @@
+"""Benchmark: triangle-RHS evaluations per second of the explicit SUPG step.
+
+Workload (default): viscous flow past a circle on an O-grid with Nr=1024,
+Ntheta=2048 (2,097,152 nodes, 4,190,208 triangles), M=0.8, Re=1000, fixed
+mesh -- the plain FP64 GPU path.  A step is one full time step of the
+reference's main loop (deltat, RK with 4 calcRHS evaluations, ESTAB, the
+two mesh-Laplace PCG solves of fluidStructure, U = U1).  value = 4 * nelem *
+steps / timed seconds (every calcRHS evaluation complete, none hoisted).
+
+The timed region starts with all inputs resident in HBM and is bracketed by
+a stream synchronisation on both sides.  Per-kernel times come from HIP
+events recorded on the session's own stream inside the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "triangle-RHS evals/s, %HBM roofline; ALE step time learned vs plain at equal tol"
+
+
+def element_bytes(nelem: int, npoin: int) -> int:
+    """Algorithmic bytes of one element-kernel launch (calcRHS element part):
+    per element inpoel 12 + dNx,dNy 48 + area,dtl,shoc,tau1..3 48 = 108 B; per
+    node U 32 + T 8 = 40 B (theta == 0 inside RK is not read)."""
+    return 108 * nelem + 40 * npoin
+
+
+def cpu_baseline(cs, seconds_budget: float = 20.0) -> dict:
+    """The oracle (C restatement, OpenMP element loops with atomic scatters,
+    -O3) on the host cores, on whole steps of the same workload."""
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    orc.set_threads(threads)
+    o = orc.OracleSolver(cs)
+    o.step()                                  # first touch / warm caches
+    t0 = time.time()
+    n = 0
+    while True:
+        o.step()
+        n += 1
+        if time.time() - t0 > seconds_budget / 2 or n >= 20:
+            break
+    dt = time.time() - t0
+    return {"value": 4.0 * cs.nelem * n / dt, "unit": "triangle-RHS evals/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} full steps (after 1 untimed) of the same {cs.nelem}-triangle case, "
+                      f"oracle/cfd_oracle.c -O3 -fopenmp, {threads} threads"}
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), committed under profiles/; None when absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.isfile(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--nr", type=int, default=1024)
+    ap.add_argument("--nt", type=int, default=2048)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU: the path is a single-GPU kernel set "
+              "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from cfd_amd.mesh import cylinder_case
+    from cfd_amd.solver import Session
+
+    cs = cylinder_case(args.nr, args.nt, mach=0.8, re=1000.0)
+    prec = 0 if args.precision == "fp64" else 1
+    S = Session(cs, precision=prec)
+    S.step(args.warmup)
+    S.sync()
+    S.timing(True)
+    S.sync()
+    t0 = time.perf_counter()
+    S.step(args.steps)
+    S.sync()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    value = 4.0 * cs.nelem * args.steps / sec
+    te, ne = S.kernel_time("ELEMENT")
+    ts, ns = S.kernel_time("STAGE")
+    avg_elem_s = te / ne * 1e-3
+    bytes_elem = element_bytes(cs.nelem, cs.npoin)
+    achieved = bytes_elem / avg_elem_s / 1e9
+    kern = {}
+    for k in ("ELEMENT", "STAGE", "ESTAB", "DELTAT", "PCG", "GEOM", "STEP"):
+        t, n = S.kernel_time(k)
+        if n:
+            kern[k.lower()] = {"avg_us": round(t / n * 1e3, 2), "launches": n}
+    sc = S.scalars()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "triangle-RHS evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else "f32-element/f64-accumulate",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cylinder O-grid Nr={args.nr} Ntheta={args.nt} ({cs.npoin} nodes, "
+                        f"{cs.nelem} triangles), M=0.8, Re=1000, fixed mesh, "
+                        f"{'plain FP64' if prec == 0 else 'FP32-element'} path",
+            "nelem": cs.nelem, "npoin": cs.npoin, "calcrhs_per_step": 4,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_element_rhs (calcRHS element residual)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": read_traffic(f"element_{args.precision}_traffic_cfg2.txt"),
+            "algorithmic_bytes_per_launch": bytes_elem,
+            "avg_launch_us": avg_elem_s * 1e6,
+        },
+        "kernels": kern,
+        "state": {"dtmin": sc["DTMIN"], "time": sc["TIME"], "fx": sc["FX1"], "fy": sc["FY1"]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cs)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

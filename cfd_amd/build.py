@@ -1,0 +1,87 @@
+"""Builds libcfd_amd.so (HIP, gfx950) in-tree, and the test-side oracle
+libraries when asked.
+
+    python -m cfd_amd.build            # library only
+    python -m cfd_amd.build --all      # + oracle/liboracle.so (+ oracle/_ref)
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libcfd_amd.so")
+SOURCES = ["tables.hip", "geom.hip", "flow.hip", "pcg.hip", "api.hip"]
+HEADERS = ["cfd_internal.hpp", "reduce.hpp", "crmath.hpp"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# -ffp-contract=off: products and sums stay separately rounded, as in the
+# reference build; packed FP32 ops are disabled for all device code.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-fno-gpu-rdc", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+]
+
+
+def _newer(target: str, deps: list[str]) -> bool:
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd[:3]) + " ...")
+    # keep warnings visible but compact
+    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+    if err.strip():
+        sys.stderr.write(err + "\n")
+
+
+def build_library(force: bool = True, jobs: int = 8) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    hdrs = [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(ROOT, "include", "cfd_abi.h")]
+    objs, todo = [], []
+    for src in SOURCES:
+        sp = os.path.join(CSRC, src)
+        op = os.path.join(OBJ, src.replace(".hip", ".o"))
+        objs.append(op)
+        if force or _newer(op, [sp] + hdrs):
+            todo.append([HIPCC, *HIP_FLAGS, "-c", sp, "-o", op])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, todo))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile"])
+
+
+def build_reference() -> bool:
+    """Compiles the reference into oracle/_ref when its sources are present."""
+    if not os.path.isdir("/root/reference") or shutil.which("flang") is None:
+        return False
+    _run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile.ref"])
+    return True
+
+
+if __name__ == "__main__":
+    build_library(force="--incremental" not in sys.argv)
+    if "--all" in sys.argv:
+        build_oracle()
+        build_reference()
+    print(LIB)

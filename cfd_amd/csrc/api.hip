@@ -1,0 +1,1029 @@
+// C ABI of libcfd_amd.so: mesh context, host-pointer entries for the
+// reference routines, and the device-resident session (time step of
+// ns2DComp.ALE.f90:138-281).  See include/cfd_abi.h for the contract.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cfd_internal.hpp"
+
+namespace cfd {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+static Consts to_consts(const cfd_consts* c) {
+    Consts k;
+    k.gama = c->gama; k.fr = c->fr; k.fmu = c->fmu; k.fk = c->fk; k.fcv = c->fcv;
+    k.t_inf = c->t_inf; k.rho_inf = c->rho_inf; k.u_inf = c->u_inf; k.v_inf = c->v_inf;
+    k.cte = c->cte; k.fsafe = c->fsafe;
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// small kernels used by the entries and the session
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void k_fill(double* __restrict__ a, double v, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+// DTL = DTMIN*f + DT*(1-f) (ns2DComp.ALE.f90:160-161)
+__global__ void k_dtl_blend(double* __restrict__ dtl, const double* __restrict__ DT, double dtmin,
+                            double f, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dtl[i] = dtmin * f + DT[i] * (1.0 - f);
+}
+// free stream (ns2DComp.ALE.f90:409-420) plus the start values of the work arrays
+__global__ void k_freestream(double* __restrict__ U, double* __restrict__ U1, double* __restrict__ VX,
+                             double* __restrict__ VY, double* __restrict__ T, double* __restrict__ GAMM,
+                             double* __restrict__ RHO, double* __restrict__ E, double* __restrict__ WX,
+                             double* __restrict__ WY, double rho, double u, double v, double t,
+                             double gama, double pamb, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    GAMM[i] = gama;
+    double energia = pamb / ((gama - 1.0) * rho) + 0.5 * (u * u + v * v);
+    double u4 = energia * rho;
+    double4 q = make_double4(rho, rho * u, rho * v, u4);
+    reinterpret_cast<double4*>(U)[i] = q;
+    reinterpret_cast<double4*>(U1)[i] = q;
+    VX[i] = u;
+    VY[i] = v;
+    T[i] = t;
+    RHO[i] = rho;
+    E[i] = u4 / rho;
+    WX[i] = -0.0;
+    WY[i] = 0.0;
+}
+// POS_AUX(1:NMOVE) = DPOS(ilaux), POS_AUX(NMOVE+1:NNMOVE) = 0 (meshMove.f90:79-89)
+__global__ void k_pos_aux(const int* __restrict__ ilaux, const double* __restrict__ dpos,
+                          double* __restrict__ pos, int nmove, int nn) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nn) return;
+    pos[q] = q < nmove ? dpos[ilaux[q] - 1] : 0.0;
+}
+// X += XPOS; X1 += XPOS; W = XPOS/DTMIN (meshMove.f90:99-104)
+__global__ void k_move(double* __restrict__ X, double* __restrict__ X1, double* __restrict__ W,
+                       const double* __restrict__ pos, double dtmin, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = pos[i];
+    X[i] = X[i] + d;
+    X1[i] = X1[i] + d;
+    W[i] = d / dtmin;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Boundary-condition table (host construction, reference list semantics)
+// ---------------------------------------------------------------------------
+struct BcHost {
+    std::vector<int> nodes, bcidx, flags;
+    std::vector<double> vals;
+    std::vector<int> wnode_bc, wnode_ptr, wnode_edges;  // wall-node incidence
+};
+
+static BcHost build_bc_host(int npoin, const cfd_bc_lists* bc) {
+    BcHost h;
+    h.bcidx.assign(npoin, -1);
+    auto entry = [&](int node) -> int {
+        CFD_CHECK(node >= 1 && node <= npoin, "boundary list node id out of range");
+        int& b = h.bcidx[node - 1];
+        if (b < 0) {
+            b = (int)h.nodes.size();
+            h.nodes.push_back(node);
+            h.flags.push_back(0);
+            h.vals.insert(h.vals.end(), 8, 0.0);
+        }
+        return b;
+    };
+    if (!bc) return h;
+    for (int i = 0; i < bc->nfixv; ++i) {           // FIXVEL: last occurrence wins
+        int b = entry(bc->ifixv_node[i]);
+        h.flags[b] |= 1;
+        h.vals[8 * b + 0] = bc->rfixv_valuex[i];
+        h.vals[8 * b + 1] = bc->rfixv_valuey[i];
+    }
+    for (int i = 0; i < bc->nfixrho; ++i) {
+        int b = entry(bc->ifixrho_node[i]);
+        h.flags[b] |= 4;
+        h.vals[8 * b + 4] = bc->rfixrho_value[i];
+    }
+    for (int i = 0; i < bc->nfixt; ++i) {
+        int b = entry(bc->ifixt_node[i]);
+        h.flags[b] |= 8;
+        h.vals[8 * b + 5] = bc->rfixt_value[i];
+    }
+    // wall nodes: incident edges in ascending edge order (NORMALES sums)
+    std::map<int, std::vector<int>> inc;
+    for (int e = 0; e < bc->nwall; ++e)
+        for (int j = 0; j < 2; ++j) inc[bc->wall[2 * e + j]].push_back(e);
+    h.wnode_ptr.push_back(0);
+    for (auto& kv : inc) {
+        int b = entry(kv.first);
+        h.wnode_bc.push_back(b);
+        for (int e : kv.second) h.wnode_edges.push_back(e);
+        h.wnode_ptr.push_back((int)h.wnode_edges.size());
+    }
+    return h;
+}
+
+struct BcDevice {
+    BcTable tab;
+    DBuf<int> wnode_bc, wnode_ptr, wnode_edges, wall;
+    int nw = 0;
+    void upload(const BcHost& h, int npoin, const cfd_bc_lists* bc, hipStream_t s) {
+        tab.nb = (int)h.nodes.size();
+        tab.bcidx.alloc(npoin);
+        tab.bcidx.upload(h.bcidx.data(), npoin, s);
+        tab.flags.alloc(std::max<size_t>(1, h.flags.size()));
+        tab.flags.upload(h.flags.data(), h.flags.size(), s);
+        tab.vals.alloc(std::max<size_t>(8, h.vals.size()));
+        tab.vals.upload(h.vals.data(), h.vals.size(), s);
+        nw = (int)h.wnode_bc.size();
+        wnode_bc.alloc(std::max(1, nw));
+        wnode_bc.upload(h.wnode_bc.data(), nw, s);
+        wnode_ptr.alloc(nw + 1);
+        wnode_ptr.upload(h.wnode_ptr.data(), nw + 1, s);
+        wnode_edges.alloc(std::max<size_t>(1, h.wnode_edges.size()));
+        wnode_edges.upload(h.wnode_edges.data(), h.wnode_edges.size(), s);
+        int nwall = bc ? bc->nwall : 0;
+        wall.alloc(std::max(2, 2 * nwall));
+        if (nwall) wall.upload(bc->wall, 2 * nwall, s);
+        CFD_HIP(hipStreamSynchronize(s));
+    }
+    void normales(const double* X, const double* Y, hipStream_t s) {
+        launch_normales(tab, nw, wnode_bc.p, wnode_ptr.p, wnode_edges.p, wall.p, X, Y, s);
+    }
+};
+
+}  // namespace cfd
+
+using namespace cfd;
+
+// ---------------------------------------------------------------------------
+// mesh context
+// ---------------------------------------------------------------------------
+struct cfd_mesh {
+    hipStream_t s = nullptr;
+    Tables t;
+    std::map<std::string, DBuf<double>> dscratch;
+    DBuf<double> partials;
+    DBuf<unsigned> counter;
+    DBuf<double> dscal;
+    // NORMALES / RK state
+    std::unique_ptr<BcDevice> bc;
+    // gcl_mod saved state
+    DBuf<double> w_x_old, w_y_old, area_old;
+    bool has_w = false, has_area = false;
+    PcgWork pcg;
+
+    double* scratch(const char* name, size_t n) {
+        auto& b = dscratch[name];
+        if (b.n < n) b.alloc(n);
+        return b.p;
+    }
+    template <class T>
+    T* put(const char* name, const T* h, size_t n) {
+        static_assert(sizeof(T) <= sizeof(double), "scratch element too large");
+        double* d = scratch(name, (n * sizeof(T) + 7) / 8 + 1);
+        if (n) CFD_HIP(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+        return reinterpret_cast<T*>(d);
+    }
+    template <class T>
+    void get(T* h, const T* d, size_t n) {
+        if (n) CFD_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    }
+    void sync() { CFD_HIP(hipStreamSynchronize(s)); }
+};
+
+#define CFD_TRY(...)                                    \
+    try {                                               \
+        __VA_ARGS__;                                    \
+        return CFD_OK;                                  \
+    } catch (const cfd::Error& e) {                     \
+        set_last_error(e.what());                       \
+        std::string w = e.what();                       \
+        if (w.find("mesh:") == 0) return CFD_ERR_MESH;  \
+        if (w.find("state:") == 0) return CFD_ERR_STATE;\
+        if (w.find("arg:") == 0) return CFD_ERR_ARG;    \
+        return CFD_ERR_DEVICE;                          \
+    } catch (const std::exception& e) {                 \
+        set_last_error(e.what());                       \
+        return CFD_ERR_DEVICE;                          \
+    }
+
+#define CFD_ARG(cond, msg) CFD_CHECK(cond, std::string("arg: ") + msg)
+
+extern "C" {
+
+int cfd_abi_version(void) { return CFD_ABI_VERSION; }
+const char* cfd_last_error(void) { return g_last_error.c_str(); }
+
+int cfd_device_count(int* count) {
+    CFD_TRY({
+        int n = 0;
+        CFD_HIP(hipGetDeviceCount(&n));
+        *count = n;
+    })
+}
+
+int cfd_mesh_create(cfd_mesh** out, int nelem, int npoin, const int* inpoel) {
+    CFD_TRY({
+        CFD_ARG(out && inpoel && nelem > 0 && npoin > 0, "mesh sizes / pointers");
+        int ndev = 0;
+        CFD_HIP(hipGetDeviceCount(&ndev));
+        CFD_CHECK(ndev > 0, "no GPU device");
+        auto m = std::make_unique<cfd_mesh>();
+        CFD_HIP(hipStreamCreate(&m->s));
+        build_tables(m->t, inpoel, nelem, npoin, m->s);
+        m->partials.alloc(grid_for(std::max(nelem, npoin)));
+        m->counter.alloc(1);
+        m->counter.zero(m->s);
+        m->dscal.alloc(8);
+        m->sync();
+        *out = m.release();
+    })
+}
+
+int cfd_mesh_destroy(cfd_mesh* m) {
+    CFD_TRY({
+        if (!m) return CFD_OK;
+        hipStream_t s = m->s;
+        (void)hipStreamSynchronize(s);
+        delete m;
+        (void)hipStreamDestroy(s);
+    })
+}
+
+int cfd_mesh_sizes(const cfd_mesh* m, int* npsup, int* nnz) {
+    CFD_TRY({
+        CFD_ARG(m, "null mesh");
+        if (npsup) *npsup = m->t.npsup;
+        if (nnz) *nnz = m->t.nnz;
+    })
+}
+
+int cfd_get_esup(cfd_mesh* m, int* esup1, int* esup2) {
+    CFD_TRY({
+        m->t.esup1.download(esup1, 3 * (size_t)m->t.nelem, m->s);
+        m->t.esup2.download(esup2, m->t.npoin + 1, m->s);
+        m->sync();
+    })
+}
+
+int cfd_get_psup(cfd_mesh* m, int* psup1, int* psup2) {
+    CFD_TRY({
+        m->t.psup1.download(psup1, m->t.npsup, m->s);
+        m->t.psup2.download(psup2, m->t.npoin + 1, m->s);
+        m->sync();
+    })
+}
+
+int cfd_lap_structure(cfd_mesh* m, int* rowptr, int* idx) {
+    CFD_TRY({
+        m->t.rowptr.download(rowptr, m->t.npoin + 1, m->s);
+        m->t.idx.download(idx, m->t.nnz, m->s);
+        m->sync();
+    })
+}
+
+int cfd_laplace(cfd_mesh* m, const double* X, const double* Y, const double* dNx, const double* dNy,
+                double* lap_sparse, double* lap_diag) {
+    CFD_TRY({
+        const int np = m->t.npoin, ne = m->t.nelem;
+        double* dX = m->put("X", X, np);
+        double* dY = m->put("Y", Y, np);
+        double* dnx = m->put("dNx", dNx, 3 * (size_t)ne);
+        double* dny = m->put("dNy", dNy, 3 * (size_t)ne);
+        double* lap = m->scratch("lap", m->t.nnz);
+        double* diag = m->scratch("diag", np);
+        launch_laplace(m->t, dX, dY, dnx, dny, lap, diag, m->s);
+        m->get(lap_sparse, lap, m->t.nnz);
+        m->get(lap_diag, diag, np);
+        m->sync();
+    })
+}
+
+int cfd_deriv(cfd_mesh* m, const double* X, const double* Y, double* area, double* dNx, double* dNy,
+              double* HH, double* HHX, double* HHY, double* hmin) {
+    CFD_TRY({
+        const int np = m->t.npoin, ne = m->t.nelem;
+        double* dX = m->put("X", X, np);
+        double* dY = m->put("Y", Y, np);
+        double* a = m->scratch("area", ne);
+        double* nx = m->scratch("dNx", 3 * (size_t)ne);
+        double* ny = m->scratch("dNy", 3 * (size_t)ne);
+        double* h = m->scratch("HH", ne);
+        double* hx = m->scratch("HHX", ne);
+        double* hy = m->scratch("HHY", ne);
+        launch_deriv(m->t, dX, dY, a, nx, ny, h, hx, hy, m->dscal.p, m->partials.p, m->counter.p,
+                     m->s);
+        m->get(area, a, ne);
+        m->get(dNx, nx, 3 * (size_t)ne);
+        m->get(dNy, ny, 3 * (size_t)ne);
+        if (HH) m->get(HH, h, ne);
+        if (HHX) m->get(HHX, hx, ne);
+        if (HHY) m->get(HHY, hy, ne);
+        if (hmin) m->get(hmin, (const double*)m->dscal.p, 1);
+        m->sync();
+    })
+}
+
+int cfd_masas(cfd_mesh* m, const double* area, double* M) {
+    CFD_TRY({
+        double* a = m->put("area", area, m->t.nelem);
+        double* dM = m->scratch("M", m->t.npoin);
+        launch_masas(m->t, a, dM, m->s);
+        m->get(M, dM, m->t.npoin);
+        m->sync();
+    })
+}
+
+int cfd_deltat(cfd_mesh* m, const cfd_consts* c, const double* area, const double* T,
+               const double* VEL_X, const double* VEL_Y, const double* W_X, const double* W_Y,
+               double* DT, double* dtmin) {
+    CFD_TRY({
+        CFD_ARG(c, "null consts");
+        const int np = m->t.npoin, ne = m->t.nelem;
+        Consts k = to_consts(c);
+        double* a = m->put("area", area, ne);
+        double* t = m->put("T", T, np);
+        double* vx = m->put("VX", VEL_X, np);
+        double* vy = m->put("VY", VEL_Y, np);
+        double* wx = m->put("WX", W_X, np);
+        double* wy = m->put("WY", W_Y, np);
+        double* dt = m->scratch("DT", ne);
+        launch_deltat(m->t, k, a, t, vx, vy, wx, wy, dt, m->dscal.p, m->partials.p, m->counter.p,
+                      m->s);
+        m->get(DT, dt, ne);
+        m->get(dtmin, (const double*)m->dscal.p, 1);
+        m->sync();
+    })
+}
+
+int cfd_estab(cfd_mesh* m, const cfd_consts* c, const double* U, const double* T, const double* GAMM,
+              const double* VEL_X, const double* VEL_Y, const double* W_X, const double* W_Y,
+              const double* dNx, const double* dNy, double dtmin, double* shoc, double* t1,
+              double* t2, double* t3) {
+    CFD_TRY({
+        CFD_ARG(c, "null consts");
+        const int np = m->t.npoin, ne = m->t.nelem;
+        Consts k = to_consts(c);
+        double* u = m->put("U", U, 4 * (size_t)np);
+        double* t = m->put("T", T, np);
+        double* g = m->put("GAMM", GAMM, np);
+        double* vx = m->put("VX", VEL_X, np);
+        double* vy = m->put("VY", VEL_Y, np);
+        double* wx = m->put("WX", W_X, np);
+        double* wy = m->put("WY", W_Y, np);
+        double* nx = m->put("dNx", dNx, 3 * (size_t)ne);
+        double* ny = m->put("dNy", dNy, 3 * (size_t)ne);
+        double* o0 = m->scratch("shoc", ne);
+        double* o1 = m->scratch("t1", ne);
+        double* o2 = m->scratch("t2", ne);
+        double* o3 = m->scratch("t3", ne);
+        launch_estab(m->t, k, u, t, g, vx, vy, wx, wy, nx, ny, dtmin, o0, o1, o2, o3, m->s);
+        m->get(shoc, o0, ne);
+        m->get(t1, o1, ne);
+        m->get(t2, o2, ne);
+        m->get(t3, o3, ne);
+        m->sync();
+    })
+}
+
+static void element_inputs(cfd_mesh* m, ElemIn& in, const double* U, const double* dNx,
+                           const double* dNy, const double* area, const double* dtl) {
+    const int np = m->t.npoin, ne = m->t.nelem;
+    in.U = m->put("U", U, 4 * (size_t)np);
+    in.dNx = m->put("dNx", dNx, 3 * (size_t)ne);
+    in.dNy = m->put("dNy", dNy, 3 * (size_t)ne);
+    in.area = m->put("area", area, ne);
+    in.dtl = m->put("dtl", dtl, ne);
+}
+
+int cfd_calcrhs(cfd_mesh* m, const cfd_consts* c, double* rhs, const double* U, const double* theta,
+                const double* T, const double* dNx, const double* dNy, const double* area,
+                const double* shoc, const double* dtl, const double* t1, const double* t2,
+                const double* t3) {
+    CFD_TRY({
+        CFD_ARG(c && rhs && U && T, "null argument");
+        const int np = m->t.npoin, ne = m->t.nelem;
+        Consts k = to_consts(c);
+        ElemIn in{};
+        element_inputs(m, in, U, dNx, dNy, area, dtl);
+        in.theta = theta ? m->put("theta", theta, 4 * (size_t)np) : nullptr;
+        in.T = m->put("T", T, np);
+        in.shoc = m->put("shoc", shoc, ne);
+        in.t1 = m->put("t1", t1, ne);
+        in.t2 = m->put("t2", t2, ne);
+        in.t3 = m->put("t3", t3, ne);
+        in.WX = in.WY = nullptr;
+        double* cc = m->scratch("c_calc", 12 * (size_t)ne);
+        double* r = m->put("rhs", rhs, 4 * (size_t)np);
+        launch_element_rhs(m->t, k, in, cc, nullptr, m->s);
+        launch_gather_rhs(m->t, r, cc, nullptr, r, m->s);
+        m->get(rhs, (const double*)r, 4 * (size_t)np);
+        m->sync();
+    })
+}
+
+int cfd_fuente(cfd_mesh* m, double* rhs, const double* U, const double* W_X, const double* W_Y,
+               const double* dNx, const double* dNy, const double* area, const double* dtl) {
+    CFD_TRY({
+        const int np = m->t.npoin, ne = m->t.nelem;
+        ElemIn in{};
+        element_inputs(m, in, U, dNx, dNy, area, dtl);
+        in.WX = m->put("WX", W_X, np);
+        in.WY = m->put("WY", W_Y, np);
+        double* cf = m->scratch("c_fuente", 12 * (size_t)ne);
+        launch_fuente(m->t, in, cf, m->s);
+        double* r = m->put("rhs", rhs, 4 * (size_t)np);
+        launch_gather_rhs(m->t, r, cf, nullptr, r, m->s);
+        m->get(rhs, (const double*)r, 4 * (size_t)np);
+        m->sync();
+    })
+}
+
+int cfd_normales(cfd_mesh* m, const cfd_bc_lists* bc, const double* X, const double* Y) {
+    CFD_TRY({
+        const int np = m->t.npoin;
+        BcHost h = build_bc_host(np, bc);
+        m->bc = std::make_unique<BcDevice>();
+        m->bc->upload(h, np, bc, m->s);
+        double* dX = m->put("X", X, np);
+        double* dY = m->put("Y", Y, np);
+        m->bc->normales(dX, dY, m->s);
+        m->sync();
+    })
+}
+
+int cfd_rk(cfd_mesh* m, const cfd_consts* c, const cfd_bc_lists* bc, int nrk, double dtmin,
+           const double* dtl, cfd_rk_nodes* nd, cfd_rk_elems* el) {
+    CFD_TRY({
+        CFD_ARG(c && nd && el && nrk >= 1, "null argument");
+        CFD_CHECK(m->bc, "state: cfd_normales must run before cfd_rk");
+        (void)bc;
+        const int np = m->t.npoin, ne = m->t.nelem;
+        const size_t n4 = 4 * (size_t)np;
+        Consts k = to_consts(c);
+        double* U = m->put("U", nd->U, n4);
+        double* T = m->put("T", nd->T, np);
+        double* VX = m->put("VX", nd->VEL_X, np);
+        double* VY = m->put("VY", nd->VEL_Y, np);
+        double* WX = m->put("WX", nd->W_X, np);
+        double* WY = m->put("WY", nd->W_Y, np);
+        double* M = m->put("M", nd->M, np);
+        double* G = m->put("GAMM", nd->GAMM, np);
+        double* nx = m->put("dNx", el->dNx, 3 * (size_t)ne);
+        double* ny = m->put("dNy", el->dNy, 3 * (size_t)ne);
+        double* ar = m->put("area", el->area, ne);
+        double* dl = m->put("dtl", dtl, ne);
+        double* sh = m->scratch("shoc", ne);
+        double* t1 = m->scratch("t1", ne);
+        double* t2 = m->scratch("t2", ne);
+        double* t3 = m->scratch("t3", ne);
+        double* U1 = m->scratch("U1", n4);
+        double* RHS = m->scratch("RHS", n4);
+        double* RHO = m->scratch("RHO", np);
+        double* E = m->scratch("E", np);
+        double* P = m->scratch("P", np);
+        double* RM = m->scratch("RMACH", np);
+        double* cc = m->scratch("c_calc", 12 * (size_t)ne);
+        double* cf = m->scratch("c_fuente", 12 * (size_t)ne);
+        // ESTAB at stage 1 with the start-of-step U and T (subrutinas.f90:671-677)
+        launch_estab(m->t, k, U, T, G, VX, VY, WX, WY, nx, ny, dtmin, sh, t1, t2, t3, m->s);
+        ElemIn in{U, nullptr, T, nx, ny, ar, sh, dl, t1, t2, t3, WX, WY};
+        StageNodes sn{U, U1, RHS, T, VX, VY, RHO, E, P, RM, M, G, WX, WY};
+        for (int irk = 1; irk <= nrk; ++irk) {
+            double rk_fact = 1.0 / (nrk + 1 - irk);
+            launch_element_rhs(m->t, k, in, cc, cf, m->s);
+            launch_stage_update(m->t, k, m->bc->tab, rk_fact, cc, cf, sn, true, m->s);
+        }
+        m->get(nd->U1, (const double*)U1, n4);
+        m->get(nd->RHS, (const double*)RHS, n4);
+        m->get(nd->T, (const double*)T, np);
+        m->get(nd->VEL_X, (const double*)VX, np);
+        m->get(nd->VEL_Y, (const double*)VY, np);
+        m->get(nd->RHO, (const double*)RHO, np);
+        m->get(nd->E, (const double*)E, np);
+        m->get(nd->P, (const double*)P, np);
+        m->get(nd->RMACH, (const double*)RM, np);
+        m->get(el->shoc, (const double*)sh, ne);
+        m->get(el->t_sugn1, (const double*)t1, ne);
+        m->get(el->t_sugn2, (const double*)t2, ne);
+        m->get(el->t_sugn3, (const double*)t3, ne);
+        m->sync();
+    })
+}
+
+int cfd_bicg(int npoin, const double* spMtx, const int* spIdx, const int* spRowptr,
+             const double* diagMtx, double* x, const double* b, const double* x_fix,
+             const int* x_fixIdx, int nfix, int* iters, double* rz) {
+    CFD_TRY({
+        CFD_ARG(npoin > 0 && spMtx && spIdx && spRowptr && diagMtx && x && b, "null argument");
+        CFD_ARG(nfix >= 0 && (nfix == 0 || (x_fix && x_fixIdx)), "fixed-node lists");
+        for (int q = 0; q < nfix; ++q)
+            CFD_ARG(x_fixIdx[q] >= 1 && x_fixIdx[q] <= npoin, "x_fixIdx out of range");
+        const int nnz = spRowptr[npoin];
+        CFD_ARG(nnz >= 0 && spRowptr[0] == 0, "row pointer");
+        for (int i = 0; i < nnz; ++i)
+            CFD_ARG(spIdx[i] >= 1 && spIdx[i] <= npoin, "column index out of range");
+        hipStream_t s;
+        CFD_HIP(hipStreamCreate(&s));
+        struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{s};
+        DBuf<double> A(nnz), D(npoin), X(npoin), B(npoin), XF(std::max(1, nfix));
+        DBuf<int> I(nnz), R(npoin + 1), FI(std::max(1, nfix));
+        A.upload(spMtx, nnz, s);
+        I.upload(spIdx, nnz, s);
+        R.upload(spRowptr, npoin + 1, s);
+        D.upload(diagMtx, npoin, s);
+        X.upload(x, npoin, s);
+        B.upload(b, npoin, s);
+        XF.upload(x_fix, nfix, s);
+        FI.upload(x_fixIdx, nfix, s);
+        PcgWork w;
+        PcgResult res = pcg_solve(w, npoin, A.p, I.p, R.p, D.p, X.p, B.p, XF.p, FI.p, nfix, 1.0e-10,
+                                  1000, 8, s);
+        X.download(x, npoin, s);
+        CFD_HIP(hipStreamSynchronize(s));
+        if (iters) *iters = res.iters;
+        if (rz) *rz = res.rz;
+    })
+}
+
+int cfd_gcl_put_w(cfd_mesh* m, const double* W_x, const double* W_y) {
+    CFD_TRY({
+        const int np = m->t.npoin;
+        if (!m->has_w) {
+            m->w_x_old.alloc(np);
+            m->w_y_old.alloc(np);
+        }
+        m->w_x_old.upload(W_x, np, m->s);
+        m->w_y_old.upload(W_y, np, m->s);
+        m->has_w = true;
+        m->sync();
+    })
+}
+
+int cfd_gcl_put_area(cfd_mesh* m, const double* area) {
+    CFD_TRY({
+        const int ne = m->t.nelem;
+        if (!m->has_area) m->area_old.alloc(ne);
+        m->area_old.upload(area, ne, m->s);
+        m->has_area = true;
+        m->sync();
+    })
+}
+
+int cfd_gcl_main(cfd_mesh* m, double* M, const double* W_x, const double* W_y, const double* dNx,
+                 const double* dNy, const double* area, double dt, int mode) {
+    CFD_TRY({
+        // gcl.f90:23-25 stops when putW/putArea have not run
+        CFD_CHECK(m->has_w && m->has_area, "state: Faltan valores (GCL): putW/putArea first");
+        const int np = m->t.npoin, ne = m->t.nelem;
+        double* dM = m->put("M", M, np);
+        double* wx = m->put("WX", W_x, np);
+        double* wy = m->put("WY", W_y, np);
+        double* nx = m->put("dNx", dNx, 3 * (size_t)ne);
+        double* ny = m->put("dNy", dNy, 3 * (size_t)ne);
+        double* a = m->put("area", area, ne);
+        launch_gcl(m->t, dM, wx, wy, m->w_x_old.p, m->w_y_old.p, nx, ny, a, m->area_old.p, dt,
+                   mode ? 1 : 0, m->s);
+        m->get(M, (const double*)dM, np);
+        m->sync();
+    })
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device-resident session: the main program's time step
+// (ns2DComp.ALE.f90:140-281) with every array in HBM.
+//
+// Per step: deltat -> DTMIN (one 8-byte read-back for the host-side
+// hysteresis and clock, :144-166) -> RK (ESTAB, then nrk x [element
+// residual, gather + stage update]) -> fluidStructure (FORCES, TRANSF, two
+// PCG mesh solves, X += XPOS, W = XPOS/DTMIN) -> on moving meshes NORMALES,
+// DERIV, MASAS (or the GCL update), laplace -> U = U1 (a buffer swap).
+//
+// The reference copies U1 = U before RK (:168-172); RK overwrites all of U1
+// in its first stage before anything reads it, so no copy is made here.
+// FUENTE adds exactly +-0 while the mesh velocity is identically zero
+// (fixed mesh, no moving nodes, W never set by the caller); it is skipped then.
+// ---------------------------------------------------------------------------
+struct cfd_session {
+    cfd_mesh* m = nullptr;
+    hipStream_t s = nullptr;
+    Consts c{};
+    cfd_case_desc d{};
+    int np = 0, ne = 0, nnz = 0;
+    std::vector<int> iset_h, ilaux_h;
+    DBuf<double> Ub[2];
+    int ucur = 0;
+    DBuf<double> RHS, T, VX, VY, WX, WY, X, Y, X1, Y1, RHO, E, P, RMACH, M, GAMM, XPOS, YPOS;
+    DBuf<double> DTL, DT, SHOC, T1, T2, T3, AREA, DNX, DNY, HH, HHX, HHY, AREA_OLD, WXO, WYO;
+    DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, BZERO, dscal, partials;
+    DBuf<int> ILAUX, ISET;
+    DBuf<unsigned> counter;
+    BcDevice bc;
+    PcgWork pcg;
+    double sc[CFD_NSCALARS] = {0};
+    double* hpin = nullptr;
+    bool w_dirty = false;
+    // timing
+    bool timing = false;
+    double tacc[CFD_NKTIMERS] = {0};
+    long long tcnt[CFD_NKTIMERS] = {0};
+    std::vector<hipEvent_t> pool;
+    hipEvent_t open_ev[CFD_NKTIMERS] = {};
+    struct Pend { hipEvent_t a, b; int slot; };
+    std::vector<Pend> pending;
+
+    double* U() { return Ub[ucur].p; }
+    double* U1() { return Ub[ucur ^ 1].p; }
+    hipEvent_t ev() {
+        if (pool.empty()) {
+            hipEvent_t e;
+            CFD_HIP(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    void tb(int slot) {
+        if (!timing) return;
+        open_ev[slot] = ev();
+        CFD_HIP(hipEventRecord(open_ev[slot], s));
+    }
+    void te(int slot) {
+        if (!timing) return;
+        hipEvent_t b = ev();
+        CFD_HIP(hipEventRecord(b, s));
+        pending.push_back({open_ev[slot], b, slot});
+    }
+    void resolve() {
+        for (auto& p : pending) {
+            CFD_HIP(hipEventSynchronize(p.b));
+            float ms = 0.f;
+            CFD_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            tacc[p.slot] += ms;
+            tcnt[p.slot] += 1;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    ~cfd_session() {
+        if (s) (void)hipStreamSynchronize(s);
+        for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+        for (auto e : pool) (void)hipEventDestroy(e);
+        if (hpin) (void)hipHostFree(hpin);
+    }
+    bool fuente_needed() const { return d.moving || d.nmove > 0 || w_dirty; }
+
+    void geometry() {  // NORMALES, DERIV, MASAS, laplace (ns2DComp.ALE.f90:88-100, 259-275)
+        const Tables& t = m->t;
+        bc.normales(X.p, Y.p, s);
+        launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p, dscal.p + 1, partials.p,
+                     counter.p, s);
+        launch_masas(t, AREA.p, M.p, s);
+        launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
+    }
+
+    double motion_angle(double time) const {
+        if (d.motion == 1) return d.motion_amp * std::sin(d.motion_omega * time);
+        const double pi = std::acos(-1.0), ampli = pi / 8.0;  // meshMove.f90:55-57,70
+        return ampli * std::sin(10.0 * time);
+    }
+
+    void mesh_solve(DBuf<double>& dpos, DBuf<double>& pos, DBuf<double>& coord, DBuf<double>& coord1,
+                    DBuf<double>& w, double dtmin, int which) {
+        const int nn = d.nmove + d.nfix_move;
+        if (nn > 0)
+            k_pos_aux<<<grid_for(nn), kBlock, 0, s>>>(ILAUX.p, dpos.p, POSAUX.p, d.nmove, nn);
+        tb(CFD_K_PCG);
+        double tol = d.pcg_tol > 0.0 ? d.pcg_tol : 1.0e-10;
+        int maxit = d.pcg_maxit > 0 ? d.pcg_maxit : 1000;
+        PcgResult r = pcg_solve(pcg, np, LAP.p, m->t.idx.p, m->t.rowptr.p, LDIAG.p, pos.p, BZERO.p,
+                                POSAUX.p, ILAUX.p, nn, tol, maxit, 8, s);
+        te(CFD_K_PCG);
+        sc[which ? CFD_S_PCG_ITY : CFD_S_PCG_ITX] = r.iters;
+        sc[which ? CFD_S_PCG_RZY : CFD_S_PCG_RZX] = r.rz;
+        k_move<<<grid_for(np), kBlock, 0, s>>>(coord.p, coord1.p, w.p, pos.p, dtmin, np);
+        CFD_HIP(hipGetLastError());
+    }
+
+    void step() {
+        const Tables& t = m->t;
+        tb(CFD_K_STEP);
+        sc[CFD_S_ITER] += 1.0;
+        // deltat and the DTMIN hysteresis (ns2DComp.ALE.f90:144-166)
+        tb(CFD_K_DELTAT);
+        launch_deltat(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p, DT.p, dscal.p, partials.p,
+                      counter.p, s);
+        te(CFD_K_DELTAT);
+        CFD_HIP(hipMemcpyAsync(hpin, dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        CFD_HIP(hipStreamSynchronize(s));
+        double dtmin = hpin[0];
+        if ((int)sc[CFD_S_BANDERA] == 1) {
+            sc[CFD_S_DTMIN1] = dtmin;
+            sc[CFD_S_BANDERA] = 2;
+        }
+        double porc = std::fabs((dtmin - sc[CFD_S_DTMIN1]) / dtmin);
+        if (100.0 * porc <= 1.0) {
+            dtmin = sc[CFD_S_DTMIN1];
+        } else {
+            sc[CFD_S_DTMIN1] = dtmin;
+            sc[CFD_S_BANDERA] = 2;
+        }
+        if (d.itlocal != 0) {
+            double f = 1.0 - std::exp(-sc[CFD_S_ITER] * 4.6 / d.itlocal);
+            k_dtl_blend<<<grid_for(ne), kBlock, 0, s>>>(DTL.p, DT.p, dtmin, f, ne);
+        } else {
+            k_fill<<<grid_for(ne), kBlock, 0, s>>>(DTL.p, dtmin, ne);
+        }
+        sc[CFD_S_DTMIN] = dtmin;
+        sc[CFD_S_TIME] = sc[CFD_S_TIME] + dtmin;
+        const double time = sc[CFD_S_TIME];
+
+        // RK (subrutinas.f90:645-849)
+        tb(CFD_K_ESTAB);
+        launch_estab(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin, SHOC.p,
+                     T1.p, T2.p, T3.p, s);
+        te(CFD_K_ESTAB);
+        const bool fu = fuente_needed();
+        ElemIn in{U(), nullptr, T.p, DNX.p, DNY.p, AREA.p, SHOC.p, DTL.p, T1.p, T2.p, T3.p,
+                  fu ? WX.p : nullptr, fu ? WY.p : nullptr};
+        StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p, WX.p,
+                      WY.p};
+        const int nrk = d.nrk > 0 ? d.nrk : 4;
+        for (int irk = 1; irk <= nrk; ++irk) {
+            double rk_fact = 1.0 / (nrk + 1 - irk);
+            tb(CFD_K_ELEMENT);
+            launch_element_rhs(t, c, in, c_calc.p, fu ? c_fuente.p : nullptr, s);
+            te(CFD_K_ELEMENT);
+            tb(CFD_K_STAGE);
+            launch_stage_update(t, c, bc.tab, rk_fact, c_calc.p, fu ? c_fuente.p : nullptr, sn,
+                                irk == nrk, s);
+            te(CFD_K_STAGE);
+        }
+
+        // fluidStructure (meshMove.f90:52-127)
+        if (d.gcl >= 0) {
+            CFD_HIP(hipMemcpyAsync(WXO.p, WX.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+            CFD_HIP(hipMemcpyAsync(WYO.p, WY.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+        }
+        if (d.nsetedges > 0)
+            launch_forces(d.nsetedges, ISET.p, X.p, Y.p, P.p, d.xref, d.yref, dscal.p + 2, s);
+        double alphav = sc[CFD_S_DISN2];
+        sc[CFD_S_DISN2] = motion_angle(time);
+        double alpha = sc[CFD_S_DISN2] - alphav, yposr = 0.0;
+        launch_transf(d.nsetedges, ISET.p, X.p, Y.p, std::cos(alpha), std::sin(alpha), yposr, d.xref,
+                      d.yref, DXPOS.p, DYPOS.p, s);
+        mesh_solve(DXPOS, XPOS, X, X1, WX, dtmin, 0);
+        mesh_solve(DYPOS, YPOS, Y, Y1, WY, dtmin, 1);
+        sc[CFD_S_BANDERA] += 1.0;
+
+        if (d.moving == 1) {  // ns2DComp.ALE.f90:259-275
+            tb(CFD_K_GEOM);
+            if (d.gcl >= 0) {
+                CFD_HIP(hipMemcpyAsync(AREA_OLD.p, AREA.p, sizeof(double) * ne,
+                                       hipMemcpyDeviceToDevice, s));
+                bc.normales(X.p, Y.p, s);
+                launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p, dscal.p + 1,
+                             partials.p, counter.p, s);
+                launch_gcl(t, M.p, WX.p, WY.p, WXO.p, WYO.p, DNX.p, DNY.p, AREA.p, AREA_OLD.p, dtmin,
+                           d.gcl, s);
+                launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
+            } else {
+                geometry();
+            }
+            te(CFD_K_GEOM);
+        }
+        ucur ^= 1;  // U = U1 (:277-281)
+        te(CFD_K_STEP);
+        CFD_HIP(hipMemcpyAsync(hpin + 1, dscal.p + 1, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+        CFD_HIP(hipStreamSynchronize(s));
+        sc[CFD_S_HMIN] = hpin[1];
+        sc[CFD_S_FX] = hpin[2];
+        sc[CFD_S_FY] = hpin[3];
+        sc[CFD_S_RM] = hpin[4];
+        resolve();
+    }
+
+    DBuf<double>* field(int f, long long& count) {
+        const long long n = np, e = ne;
+        switch (f) {
+            case CFD_F_U: count = 4 * n; return &Ub[ucur];
+            case CFD_F_U1: count = 4 * n; return &Ub[ucur ^ 1];
+            case CFD_F_RHS: count = 4 * n; return &RHS;
+            case CFD_F_T: count = n; return &T;
+            case CFD_F_VEL_X: count = n; return &VX;
+            case CFD_F_VEL_Y: count = n; return &VY;
+            case CFD_F_W_X: count = n; return &WX;
+            case CFD_F_W_Y: count = n; return &WY;
+            case CFD_F_X: count = n; return &X;
+            case CFD_F_Y: count = n; return &Y;
+            case CFD_F_X1: count = n; return &X1;
+            case CFD_F_Y1: count = n; return &Y1;
+            case CFD_F_RHO: count = n; return &RHO;
+            case CFD_F_E: count = n; return &E;
+            case CFD_F_P: count = n; return &P;
+            case CFD_F_RMACH: count = n; return &RMACH;
+            case CFD_F_M: count = n; return &M;
+            case CFD_F_GAMM: count = n; return &GAMM;
+            case CFD_F_XPOS: count = n; return &XPOS;
+            case CFD_F_YPOS: count = n; return &YPOS;
+            case CFD_F_DTL: count = e; return &DTL;
+            case CFD_F_DT: count = e; return &DT;
+            case CFD_F_SHOC: count = e; return &SHOC;
+            case CFD_F_T_SUGN1: count = e; return &T1;
+            case CFD_F_T_SUGN2: count = e; return &T2;
+            case CFD_F_T_SUGN3: count = e; return &T3;
+            case CFD_F_AREA: count = e; return &AREA;
+            case CFD_F_DNX: count = 3 * e; return &DNX;
+            case CFD_F_DNY: count = 3 * e; return &DNY;
+            case CFD_F_HH: count = e; return &HH;
+            case CFD_F_HHX: count = e; return &HHX;
+            case CFD_F_HHY: count = e; return &HHY;
+            case CFD_F_LAP_SPARSE: count = nnz; return &LAP;
+            case CFD_F_LAP_DIAG: count = n; return &LDIAG;
+            default: throw Error("arg: unknown field id");
+        }
+    }
+};
+
+extern "C" {
+
+int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
+    CFD_TRY({
+        CFD_ARG(out && m && d && d->X && d->Y, "null argument");
+        CFD_ARG(d->nmove >= 0 && d->nfix_move >= 0 && d->nsetedges >= 0, "list sizes");
+        CFD_ARG(d->nmove + d->nfix_move == 0 || d->ilaux, "ilaux");
+        CFD_ARG(d->nsetedges == 0 || d->iset, "iset");
+        CFD_ARG(d->precision == 0, "only the FP64 element kernel is available in this build");
+        auto S = std::make_unique<cfd_session>();
+        S->m = m;
+        S->s = m->s;
+        S->d = *d;
+        S->c = to_consts(&d->c);
+        const int np = m->t.npoin, ne = m->t.nelem;
+        S->np = np;
+        S->ne = ne;
+        S->nnz = m->t.nnz;
+        const int nn = d->nmove + d->nfix_move;
+        for (int q = 0; q < nn; ++q)
+            CFD_ARG(d->ilaux[q] >= 1 && d->ilaux[q] <= np, "ilaux node out of range");
+        for (int q = 0; q < d->nsetedges; ++q)
+            for (int j = 0; j < 2; ++j)
+                CFD_ARG(d->iset[3 * q + j] >= 1 && d->iset[3 * q + j] <= np, "iset node out of range");
+        S->ilaux_h.assign(d->ilaux, d->ilaux + nn);
+        S->iset_h.assign(d->iset, d->iset + 3 * (size_t)d->nsetedges);
+        S->d.ilaux = nullptr;
+        S->d.iset = nullptr;
+        S->d.X = S->d.Y = nullptr;
+        S->d.bc = cfd_bc_lists{};
+        hipStream_t s = S->s;
+        for (auto* b : {&S->Ub[0], &S->Ub[1], &S->RHS}) b->alloc(4 * (size_t)np);
+        for (auto* b : {&S->T, &S->VX, &S->VY, &S->WX, &S->WY, &S->X, &S->Y, &S->X1, &S->Y1, &S->RHO,
+                        &S->E, &S->P, &S->RMACH, &S->M, &S->GAMM, &S->XPOS, &S->YPOS, &S->WXO,
+                        &S->WYO, &S->LDIAG, &S->DXPOS, &S->DYPOS, &S->BZERO})
+            b->alloc(np);
+        for (auto* b : {&S->DTL, &S->DT, &S->SHOC, &S->T1, &S->T2, &S->T3, &S->AREA, &S->HH, &S->HHX,
+                        &S->HHY, &S->AREA_OLD})
+            b->alloc(ne);
+        S->DNX.alloc(3 * (size_t)ne);
+        S->DNY.alloc(3 * (size_t)ne);
+        S->LAP.alloc(m->t.nnz);
+        S->c_calc.alloc(12 * (size_t)ne);
+        S->c_fuente.alloc(12 * (size_t)ne);
+        S->POSAUX.alloc(std::max(1, nn));
+        S->ILAUX.alloc(std::max(1, nn));
+        S->ILAUX.upload(S->ilaux_h.data(), nn, s);
+        S->ISET.alloc(std::max<size_t>(3, S->iset_h.size()));
+        S->ISET.upload(S->iset_h.data(), S->iset_h.size(), s);
+        S->dscal.alloc(8);
+        S->dscal.zero(s);
+        S->partials.alloc(grid_for(std::max(ne, np)));
+        S->counter.alloc(1);
+        S->counter.zero(s);
+        // zero everything once (DXPOS/DYPOS keep zeros off the set nodes,
+        // meshMove.f90:52-53; B stays zero, :91-95)
+        for (auto* b : {&S->Ub[0], &S->Ub[1], &S->RHS, &S->T, &S->VX, &S->VY, &S->WX, &S->WY, &S->X1,
+                        &S->Y1, &S->RHO, &S->E, &S->P, &S->RMACH, &S->M, &S->GAMM, &S->XPOS, &S->YPOS,
+                        &S->WXO, &S->WYO, &S->LDIAG, &S->DXPOS, &S->DYPOS, &S->BZERO, &S->DTL, &S->DT,
+                        &S->SHOC, &S->T1, &S->T2, &S->T3, &S->AREA, &S->HH, &S->HHX, &S->HHY,
+                        &S->AREA_OLD, &S->DNX, &S->DNY, &S->LAP})
+            b->zero(s);
+        S->X.upload(d->X, np, s);
+        S->Y.upload(d->Y, np, s);
+        BcHost h = build_bc_host(np, &d->bc);
+        S->bc.upload(h, np, &d->bc, s);
+        S->pcg.ensure(np);
+        CFD_HIP(hipHostMalloc(&S->hpin, 16 * sizeof(double)));
+        S->sc[CFD_S_BANDERA] = 1.0;
+        CFD_HIP(hipStreamSynchronize(s));
+        *out = S.release();
+    })
+}
+
+int cfd_session_destroy(cfd_session* S) {
+    CFD_TRY({ delete S; })
+}
+
+int cfd_session_init_freestream(cfd_session* S) {
+    CFD_TRY({
+        CFD_ARG(S, "null session");
+        const Consts& c = S->c;
+        const int np = S->np;
+        double pamb = c.rho_inf * c.fr * c.t_inf;
+        k_freestream<<<grid_for(np), kBlock, 0, S->s>>>(S->U(), S->U1(), S->VX.p, S->VY.p, S->T.p,
+                                                        S->GAMM.p, S->RHO.p, S->E.p, S->WX.p, S->WY.p,
+                                                        c.rho_inf, c.u_inf, c.v_inf, c.t_inf, c.gama,
+                                                        pamb, np);
+        CFD_HIP(hipGetLastError());
+        S->geometry();
+        CFD_HIP(hipMemcpyAsync(S->AREA_OLD.p, S->AREA.p, sizeof(double) * S->ne,
+                               hipMemcpyDeviceToDevice, S->s));
+        CFD_HIP(hipMemcpyAsync(S->hpin + 1, S->dscal.p + 1, sizeof(double), hipMemcpyDeviceToHost, S->s));
+        CFD_HIP(hipStreamSynchronize(S->s));
+        S->sc[CFD_S_HMIN] = S->hpin[1];
+        for (int k : {CFD_S_TIME, CFD_S_ITER, CFD_S_DTMIN, CFD_S_DTMIN1, CFD_S_DISN2}) S->sc[k] = 0.0;
+        S->sc[CFD_S_BANDERA] = 1.0;
+        S->w_dirty = false;
+    })
+}
+
+int cfd_session_step(cfd_session* S, int nsteps) {
+    CFD_TRY({
+        CFD_ARG(S && nsteps >= 0, "arguments");
+        for (int k = 0; k < nsteps; ++k) S->step();
+    })
+}
+
+int cfd_session_sync(cfd_session* S) {
+    CFD_TRY({
+        CFD_HIP(hipStreamSynchronize(S->s));
+        S->resolve();
+    })
+}
+
+int cfd_session_field_size(cfd_session* S, int f, long long* count) {
+    CFD_TRY({ S->field(f, *count); })
+}
+
+int cfd_session_get_field(cfd_session* S, int f, double* host) {
+    CFD_TRY({
+        long long n = 0;
+        DBuf<double>* b = S->field(f, n);
+        b->download(host, n, S->s);
+        CFD_HIP(hipStreamSynchronize(S->s));
+    })
+}
+
+int cfd_session_set_field(cfd_session* S, int f, const double* host) {
+    CFD_TRY({
+        long long n = 0;
+        DBuf<double>* b = S->field(f, n);
+        b->upload(host, n, S->s);
+        CFD_HIP(hipStreamSynchronize(S->s));
+        if (f == CFD_F_W_X || f == CFD_F_W_Y || f == CFD_F_XPOS || f == CFD_F_YPOS) S->w_dirty = true;
+    })
+}
+
+int cfd_session_get_scalars(cfd_session* S, double* out) {
+    CFD_TRY({ std::memcpy(out, S->sc, sizeof(S->sc)); })
+}
+
+int cfd_session_set_scalars(cfd_session* S, const double* in) {
+    CFD_TRY({ std::memcpy(S->sc, in, sizeof(S->sc)); })
+}
+
+int cfd_session_timing(cfd_session* S, int enable) {
+    CFD_TRY({
+        CFD_HIP(hipStreamSynchronize(S->s));
+        S->resolve();
+        S->timing = enable != 0;
+        for (int k = 0; k < CFD_NKTIMERS; ++k) {
+            S->tacc[k] = 0.0;
+            S->tcnt[k] = 0;
+        }
+    })
+}
+
+int cfd_session_kernel_time(cfd_session* S, int slot, double* total_ms, long long* launches) {
+    CFD_TRY({
+        CFD_ARG(slot >= 0 && slot < CFD_NKTIMERS, "timer slot");
+        *total_ms = S->tacc[slot];
+        *launches = S->tcnt[slot];
+    })
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// Internal declarations shared by the gfx950 kernels and the C-ABI layer.
+//
+// Layout in HBM (one copy per mesh / session, all device-resident):
+//   nodal   U, U1, RHS          double4 per node (the reference's U(4,npoin))
+//           T, VEL_X, ... , M   one double per node
+//   element inpoel              3 x int32 per element, 1-based node ids
+//           dNx, dNy            3 doubles per element (dNx(3,nelem))
+//           area, dtl, shoc ... one double per element
+//   tables  esup1/esup2, psup1/psup2, lap_rowptr/lap_idx as in the reference
+//           (1-based ids, 0-based offsets), plus the slot maps
+//           epos(3,nelem): index in esup1 of element e in node inpoel(k,e)'s list
+//           lslot(3, 3*nelem): CSR position of (node of esup entry s, inpoel(j,e))
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cfd_abi.h"
+
+namespace cfd {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void set_last_error(const std::string& msg);
+
+#define CFD_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw ::cfd::Error(std::string(#call) + ": " + hipGetErrorString(e_) +      \
+                               " (" __FILE__ ":" + std::to_string(__LINE__) + ")");    \
+    } while (0)
+
+#define CFD_CHECK(cond, msg)                                                            \
+    do {                                                                               \
+        if (!(cond)) throw ::cfd::Error(msg);                                          \
+    } while (0)
+
+// Owned device allocation.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) CFD_HIP(hipMalloc(&p, sizeof(T) * count));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        CFD_CHECK(count <= n, "upload larger than buffer");
+        if (count) CFD_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    }
+    void download(T* h, size_t count, hipStream_t s) const {
+        CFD_CHECK(count <= n, "download larger than buffer");
+        if (count) CFD_HIP(hipMemcpyAsync(h, p, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+    }
+    void zero(hipStream_t s) {
+        if (n) CFD_HIP(hipMemsetAsync(p, 0, sizeof(T) * n, s));
+    }
+};
+
+constexpr int kBlock = 256;
+inline unsigned grid_for(size_t n, int block = kBlock) {
+    size_t g = (n + block - 1) / block;
+    return (unsigned)(g ? g : 1);
+}
+
+// ---------------------------------------------------------------------------
+// Mesh tables (built on the device by tables.hip)
+// ---------------------------------------------------------------------------
+struct Tables {
+    int npoin = 0, nelem = 0, npsup = 0, nnz = 0;
+    DBuf<int> inpoel;            // 3*nelem
+    DBuf<int> esup1, esup2;      // 3*nelem, npoin+1
+    DBuf<int> epos;              // 3*nelem
+    DBuf<int> psup1, psup2;      // npsup, npoin+1
+    DBuf<int> rowptr, idx;       // npoin+1, nnz
+    DBuf<int> lslot;             // 9*nelem (3 per esup entry)
+};
+
+void build_tables(Tables& t, const int* inpoel_host, int nelem, int npoin, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (geometry, flow, PCG). All take device pointers and run on
+// stream s.  Names follow the reference routine each one implements.
+// ---------------------------------------------------------------------------
+struct Consts {
+    double gama, fr, fmu, fk, fcv, t_inf, rho_inf, u_inf, v_inf, cte, fsafe;
+};
+
+// deriv (subrutinas.f90:88-126); hmin_out: device double, written by the last block
+void launch_deriv(const Tables& t, const double* X, const double* Y, double* area, double* dNx,
+                  double* dNy, double* HH, double* HHX, double* HHY, double* hmin_out,
+                  double* partials, unsigned* counter, hipStream_t s);
+// MASAS (subrutinas.f90:128-153) as an ascending-element gather
+void launch_masas(const Tables& t, const double* area, double* M, hipStream_t s);
+// mLaplace.f90:7-58 values (structure comes from Tables)
+void launch_laplace(const Tables& t, const double* X, const double* Y, const double* dNx,
+                    const double* dNy, double* lap, double* diag, hipStream_t s);
+// gcl.f90:8-46 main; mode 0 = as written, 1 = corrected y-divergence
+void launch_gcl(const Tables& t, double* M, const double* Wx, const double* Wy,
+                const double* Wxo, const double* Wyo, const double* dNx, const double* dNy,
+                const double* area, const double* area_old, double dt, int mode, hipStream_t s);
+
+// deltat (subrutinas.f90:155-218): DT and dtmin (device), clipped at 10*dtmin
+void launch_deltat(const Tables& t, const Consts& c, const double* area, const double* T,
+                   const double* VX, const double* VY, const double* WX, const double* WY,
+                   double* DT, double* dtmin_out, double* partials, unsigned* counter,
+                   hipStream_t s);
+// ESTAB (subrutinas.f90:332-446)
+void launch_estab(const Tables& t, const Consts& c, const double* U, const double* T,
+                  const double* GAMM, const double* VX, const double* VY, const double* WX,
+                  const double* WY, const double* dNx, const double* dNy, double dtmin,
+                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s);
+
+// Element residual: calcRHS (calcRHS.f90:36-151) and, when WX != nullptr,
+// FUENTE (subrutinas.f90:1060-1091), written to per-slot buffers indexed
+// like esup1 (contribution of element e to node inpoel(k,e) at epos(k,e)).
+struct ElemIn {
+    const double *U, *theta, *T, *dNx, *dNy, *area, *shoc, *dtl, *t1, *t2, *t3, *WX, *WY;
+};
+void launch_element_rhs(const Tables& t, const Consts& c, const ElemIn& in, double* c_calc,
+                        double* c_fuente, hipStream_t s);
+
+// FUENTE alone into its slot buffer (uses U, dNx, dNy, area, dtl, WX, WY)
+void launch_fuente(const Tables& t, const ElemIn& in, double* c_fuente, hipStream_t s);
+
+// Deterministic node gather: rhs(i) = rhs_in(i) + sum calc contributions in
+// esup order, then + sum FUENTE contributions (reference accumulation order).
+void launch_gather_rhs(const Tables& t, const double* rhs_in, const double* c_calc,
+                       const double* c_fuente, double* rhs_out, hipStream_t s);
+
+// Boundary-condition table: unique nodes with their final actions in the
+// reference order FIXVEL -> NORMALVEL -> FIX(rho) -> FIX(T).
+struct BcTable {
+    int nb = 0;
+    DBuf<int> bcidx;     // npoin: entry in table or -1
+    DBuf<int> flags;     // nb: bit0 vel, bit1 normal, bit2 rho, bit3 T
+    DBuf<double> vals;   // nb*8: vx, vy, nx, ny, rho, T, 0, 0
+};
+
+// One RK stage node update (subrutinas.f90:695-699, 707-718, 809-826) fused
+// with the gather.  write_rhs: also store the stage RHS.
+struct StageNodes {
+    const double* U;
+    double *U1, *RHS, *T, *VX, *VY, *RHO, *E, *P, *RMACH;
+    const double *M, *GAMM, *WX, *WY;
+};
+void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
+                         const double* c_calc, const double* c_fuente, const StageNodes& nd,
+                         bool write_rhs, hipStream_t s);
+
+// NORMALES (subrutinas.f90:7-64) on the BC table entries that are wall nodes
+void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, const int* wnode_ptr,
+                     const int* wnode_edges, const int* wall, const double* X, const double* Y,
+                     hipStream_t s);
+
+// meshMove.f90 FORCES / TRANSF / the node updates of fluidStructure
+void launch_forces(int nedge, const int* iset, const double* X, const double* Y, const double* P,
+                   double xref, double yref, double* out3, hipStream_t s);
+void launch_transf(int nedge, const int* iset, const double* X, const double* Y, double ca,
+                   double sa, double yposr, double xref, double yref, double* DXPOS,
+                   double* DYPOS, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// PCG (biconjGrad.f90:8-62)
+// ---------------------------------------------------------------------------
+struct PcgWork {
+    int n = 0, nblocks = 0;
+    DBuf<double> r, z, y, pa, pb, partial;
+    DBuf<double> scal;         // see pcg.hip for the slot meaning
+    DBuf<unsigned> counter;
+    DBuf<int> fixflag;         // n: 1 on Dirichlet nodes
+    void ensure(int npoin);
+};
+struct PcgResult {
+    int iters;
+    double rz;
+    int converged;
+};
+// x (device) in/out; b (device); x_fix/x_fix_idx device arrays of nfix.
+PcgResult pcg_solve(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
+                    const double* diag, double* x, const double* b, const double* x_fix,
+                    const int* x_fix_idx, int nfix, double tol, int maxit, int check_every,
+                    hipStream_t s);
+
+}  // namespace cfd

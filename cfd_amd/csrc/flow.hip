@@ -1,0 +1,558 @@
+// Flow kernels of the explicit SUPG step.
+//
+//  k_deltat / k_deltat_clip  subrutinas.f90:155-218   element dt, global min, 10x clip
+//  k_estab                   subrutinas.f90:332-446   tau_SUGN1..3 and shock delta
+//  k_element_rhs             calcRHS.f90:36-151 (+ FUENTE subrutinas.f90:1060-1091)
+//  k_gather_rhs              the atomic scatters of calcRHS/FUENTE as a gather
+//  k_stage                   gather + RK stage update + primitives + BCs
+//                            (subrutinas.f90:679-826)
+//
+// Element work is written per slot (esup order) instead of scattered with
+// atomics; each node then sums its own contiguous slot segment, calcRHS
+// contributions first and FUENTE contributions after, in ascending element
+// order -- the order of the reference's sequential scatter.  Everything is
+// deterministic run to run.
+//
+// Floating point follows the reference expression by expression: this file
+// is compiled with -ffp-contract=off, x**2 is x*x, x**1.5d0 / x**(-.5d0) are
+// correctly rounded pow_3_2 / pow_m1_2 (crmath.hpp), x**.5d0 is sqrt().
+#include "cfd_internal.hpp"
+#include "crmath.hpp"
+#include "reduce.hpp"
+
+namespace cfd {
+namespace {
+
+// ---------------------------------------------------------------------------
+// deltat
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_deltat(const int* __restrict__ inpoel, const double* __restrict__ area, const double* __restrict__ T,
+         const double* __restrict__ VX, const double* __restrict__ VY, const double* __restrict__ WX,
+         const double* __restrict__ WY, double* __restrict__ DT, int nelem, double t_inf,
+         double fsafe, double* __restrict__ dtmin_out, double* __restrict__ partials,
+         unsigned* __restrict__ counter) {
+    double dmin = 1.0e20;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += gridDim.x * blockDim.x) {
+        const int* ip = inpoel + 3 * e;
+        double vumax = 0.0, vvmax = 0.0;
+        int p0 = ip[0] - 1, p1 = ip[1] - 1, p2 = ip[2] - 1;
+        double t_iel = (T[p0] + T[p1] + T[p2]) / 3.0;
+        int pp[3] = {p0, p1, p2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            int p = pp[i];
+            double vu = fabs(VX[p] - WX[p]);
+            double vv = fabs(VY[p] - WY[p]);
+            if (vu > vumax) vumax = vu;
+            if (vv > vvmax) vvmax = vv;
+        }
+        double hh = sqrt(2.0 * area[e]);
+        double vel = sqrt(vumax * vumax + vvmax * vvmax);
+        double smu = 110.0;
+        double fmu = 0.017 * pow_3_2(t_iel / t_inf) * (t_inf + smu) / (t_iel + smu);
+        double et = fmu;
+        double pe = (vel * hh) / (2.0 * et);
+        double alpha = pe / 3.0 < 1.0 ? pe / 3.0 : 1.0;
+        double deltatu = 1.0 / (4.0 * et / (hh * hh) + alpha * vel / hh);
+        double deltatc = 1.0 / (4.0 * et / (hh * hh));
+        double dtelem = fsafe / (1.0 / deltatc + 1.0 / deltatu);
+        DT[e] = dtelem;
+        if (dtelem < dmin) dmin = dtelem;
+    }
+    if (block_partial(dmin, OpMin(), partials, counter)) {
+        double m = combine_partials(partials, gridDim.x, 1.0e20, OpMin());
+        if (threadIdx.x == 0) {
+            *dtmin_out = m;
+            *counter = 0;
+        }
+    }
+}
+
+__global__ void k_deltat_clip(double* __restrict__ DT, const double* __restrict__ dtmin, int nelem) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    double cota = 10.0 * (*dtmin);
+    if (DT[e] > cota) DT[e] = cota;
+}
+
+// ---------------------------------------------------------------------------
+// ESTAB
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const double* __restrict__ T,
+        const double* __restrict__ GAMM, const double* __restrict__ VEL_X,
+        const double* __restrict__ VEL_Y, const double* __restrict__ W_X,
+        const double* __restrict__ W_Y, const double* __restrict__ dNx,
+        const double* __restrict__ dNy, int nelem, double fr, double tinf, double rhoinf,
+        double dtmin, double* __restrict__ shoc, double* __restrict__ t1, double* __restrict__ t2,
+        double* __restrict__ t3) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
+    double DNX[3] = {dNx[3 * e], dNx[3 * e + 1], dNx[3 * e + 2]};
+    double DNY[3] = {dNy[3 * e], dNy[3 * e + 1], dNy[3 * e + 2]};
+    double gm = (GAMM[n1] + GAMM[n2] + GAMM[n3]) / 3.0;
+    double tau = 0.0, h_rgne = 0.0, h_rgn = 0.0, h_jgn = 0.0;
+    double r1 = U[4 * n1], r2 = U[4 * n2], r3 = U[4 * n3];
+    double rho_elem = (r1 + r2 + r3) / 3.0;
+    double vx = (VEL_X[n1] + VEL_X[n2] + VEL_X[n3]) / 3.0;
+    double vy = (VEL_Y[n1] + VEL_Y[n2] + VEL_Y[n3]) / 3.0;
+    double wx = (W_X[n1] + W_X[n2] + W_X[n3]) / 3.0;
+    double wy = (W_Y[n1] + W_Y[n2] + W_Y[n3]) / 3.0;
+    vx = vx - wx;
+    vy = vy - wy;
+    double vel2 = sqrt(vx * vx + vy * vy);
+    double drx = r1 * DNX[0] + r2 * DNX[1] + r3 * DNX[2];
+    double dry = r1 * DNY[0] + r2 * DNY[1] + r3 * DNY[2];
+    double dr2 = sqrt(drx * drx + dry * dry) + 1.0e-20;
+    double T1 = T[n1], T2 = T[n2], T3 = T[n3];
+    double dtx = T1 * DNX[0] + T2 * DNX[1] + T3 * DNX[2];
+    double dty = T1 * DNY[0] + T2 * DNY[1] + T3 * DNY[2];
+    double dt2 = sqrt(dtx * dtx + dty * dty) + 1.0e-20;
+    // "velocity gradient" of an element constant: the rounding residue of
+    // sum(dN) sets the direction (reproduced as written)
+    double dux = vel2 * DNX[0] + vel2 * DNX[1] + vel2 * DNX[2];
+    double duy = vel2 * DNY[0] + vel2 * DNY[1] + vel2 * DNY[2];
+    double du2 = sqrt(dux * dux + duy * duy) + 1.0e-20;
+    double rtx = dtx / dt2, rty = dty / dt2;
+    double rjx = drx / dr2, rjy = dry / dr2;
+    double rux = dux / du2, ruy = duy / du2;
+    double temp = (T1 + T2 + T3) / 3.0;
+    double cc = sqrt(gm * fr * temp);
+    double smu = 110.0;
+    double fmu = 0.017 * pow_3_2(temp / tinf) * (tinf + smu) / (temp + smu);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double term_1 = fabs(vx * DNX[i] + vy * DNY[i]);
+        double term_2 = fabs(rjx * DNX[i] + rjy * DNY[i]);
+        double h_rgn1 = fabs(rtx * DNX[i] + rty * DNY[i]);
+        double h_rgn2 = fabs(rux * DNX[i] + ruy * DNY[i]);
+        tau = tau + term_1 + term_2 * cc;
+        h_rgne = h_rgne + h_rgn1;
+        h_rgn = h_rgn + h_rgn2;
+        h_jgn = h_jgn + term_2;
+    }
+    tau = 1.0 / tau;
+    h_rgne = 2.0 / h_rgne;
+    h_rgn = 2.0 / h_rgn;
+    if (h_rgn > 10.0) h_rgn = 0.0;
+    h_jgn = 2.0 / h_jgn;
+    if (h_jgn > 10.0) h_jgn = 0.0;
+    double tr1 = dr2 * h_jgn / rho_elem;
+    double zzz = h_jgn / (2.0 * cc);
+    shoc[e] = (tr1 + tr1 * tr1) * 0.5 * (cc * cc) * zzz;
+    double resumen = 1.0 / (tau * tau) + (2.0 / dtmin) * (2.0 / dtmin);
+    double rrr = pow_m1_2(resumen);
+    double s2 = rrr, s3 = rrr;
+    if (fmu != 0.0) {
+        double ts3 = (h_rgn * h_rgn) / (4.0 * fmu / rhoinf);
+        double ts3e = (h_rgne * h_rgne) / (4.0 * fmu / rhoinf);
+        s2 = pow_m1_2(resumen + 1.0 / (ts3 * ts3));
+        s3 = pow_m1_2(resumen + 1.0 / (ts3e * ts3e));
+    }
+    t1[e] = rrr;
+    t2[e] = s2;
+    t3[e] = s3;
+}
+
+// ---------------------------------------------------------------------------
+// SUPG element residual
+// ---------------------------------------------------------------------------
+struct ElemConsts {
+    double g, Cv, lambda_ref, mu_ref, T_inf, cte;
+    int viscous;
+};
+
+__device__ inline double4 ld4(const double* p) { return *reinterpret_cast<const double4*>(p); }
+__device__ inline void st4(double* p, double a, double b, double c, double d) {
+    *reinterpret_cast<double4*>(p) = make_double4(a, b, c, d);
+}
+
+// One element: calcRHS.f90:37-141 into rt[node][eq], unscaled order kept.
+template <bool kTheta>
+__device__ inline void supg_element(const ElemConsts& k, const double (&Uv)[3][4],
+                                   const double (&Th)[3][4], const double (&Tn)[3],
+                                   const double (&Nx)[3], const double (&Ny)[3], double tau1,
+                                   double tau2, double tau3, double shoc, double area, double dtl,
+                                   double (&rt)[3][4]) {
+    const double g = k.g;
+    double Ux[4], Uy[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        Ux[q] = Uv[0][q] * Nx[0] + Uv[1][q] * Nx[1] + Uv[2][q] * Nx[2];
+        Uy[q] = Uv[0][q] * Ny[0] + Uv[1][q] * Ny[1] + Uv[2][q] * Ny[2];
+    }
+    const double tau[3] = {tau1, tau2, tau3};
+    double nu = shoc * k.cte;
+    double T_avg = (Tn[0] + Tn[1] + Tn[2]) / 3.0;
+    double pw = pow_3_2(T_avg / k.T_inf);
+    double mu = k.mu_ref * pw * (k.T_inf + 110.0) / (T_avg + 110.0);
+    double lambda = k.lambda_ref * pw * (k.T_inf + 194.0) / (T_avg + 194.0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rt[j][q] = 0.0;
+    // quadrature points in a rolled loop: bounds the live registers of the
+    // FP64 element (occupancy) without changing any operation
+#pragma unroll 1
+    for (int kq = 0; kq < 3; ++kq) {
+        // N(j,k) = 0 for j == k, 1/2 otherwise (midpoint of the edge opposite node k)
+        double Nw[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Nw[j] = (j == kq) ? 0.0 : 0.5;
+        double Uk[4], th[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Uk[q] = Nw[0] * Uv[0][q] + Nw[1] * Uv[1][q] + Nw[2] * Uv[2][q];
+            if (kTheta)
+                th[q] = Nw[0] * Th[0][q] + Nw[1] * Th[1][q] + Nw[2] * Th[2][q];
+            else
+                th[q] = 0.0;
+        }
+        double rho = Uk[0];
+        double v1 = Uk[1] / rho;
+        double v2 = Uk[2] / rho;
+        double en = Uk[3] / rho;
+        double V_sq = v1 * v1 + v2 * v2;
+        double A[4];
+        A[0] = Ux[1] + Uy[2];
+        A[1] = 0.5 * Ux[0] * (V_sq * (g - 1.0) - 2.0 * (v1 * v1)) - Ux[1] * v1 * (g - 3.0) -
+               Ux[2] * v2 * (g - 1.0) + Ux[3] * (g - 1.0) - Uy[0] * v1 * v2 + Uy[1] * v2 + Uy[2] * v1;
+        A[2] = -(Ux[0] * v1 * v2) + Ux[1] * v2 + Ux[2] * v1 +
+               0.5 * Uy[0] * (V_sq * (g - 1.0) - 2.0 * (v2 * v2)) - Uy[1] * v1 * (g - 1.0) -
+               Uy[2] * v2 * (g - 3.0) + Uy[3] * (g - 1.0);
+        A[3] = Ux[0] * v1 * (V_sq * (g - 1.0) - en * g) -
+               0.5 * Ux[1] * (V_sq * (g - 1.0) - 2.0 * en * g + 2.0 * (v1 * v1) * (g - 1.0)) -
+               Ux[2] * v1 * v2 * (g - 1.0) + Ux[3] * g * v1 + Uy[0] * v2 * (V_sq * (g - 1.0) - en * g) -
+               Uy[1] * v1 * v2 * (g - 1.0) -
+               0.5 * Uy[2] * (V_sq * (g - 1.0) - 2.0 * en * g + 2.0 * (v2 * v2) * (g - 1.0)) +
+               Uy[3] * g * v2;
+        double a[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = th[q] + A[q];
+        double B1[4], B2[4];
+        B1[0] = a[1];
+        B1[1] = v1 * (-g + 3.0) * a[1] - v2 * (g - 1.0) * a[2] + (g - 1.0) * a[3] +
+                (0.5 * V_sq * (g - 1.0) - (v1 * v1)) * a[0];
+        B1[2] = -(v1 * v2 * a[0]) + v1 * a[2] + v2 * a[1];
+        B1[3] = g * v1 * a[3] - v1 * v2 * (g - 1.0) * a[2] + v1 * (V_sq * (g - 1.0) - en * g) * a[0] +
+                (-(0.5 * V_sq * (g - 1.0)) + en * g - (v1 * v1) * (g - 1.0)) * a[1];
+        B2[0] = a[2];
+        B2[1] = -(v1 * v2 * a[0]) + v1 * a[2] + v2 * a[1];
+        B2[2] = -(v1 * (g - 1.0) * a[1]) + v2 * (-g + 3.0) * a[2] + (g - 1.0) * a[3] +
+                (0.5 * V_sq * (g - 1.0) - (v2 * v2)) * a[0];
+        B2[3] = g * v2 * a[3] - v1 * v2 * (g - 1.0) * a[1] + v2 * (V_sq * (g - 1.0) - en * g) * a[0] +
+                (-(0.5 * V_sq * (g - 1.0)) + en * g - (v2 * v2) * (g - 1.0)) * a[2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                rt[j][q] = rt[j][q] + Nw[j] * A[q] + tau[j] * (Nx[j] * B1[q] + Ny[j] * B2[q]) +
+                           nu * (Nx[j] * Ux[q] + Ny[j] * Uy[q]);
+        if (k.viscous) {
+            const double Cv = k.Cv;
+            double K1[4], K2[4];
+            K1[1] = (2.0 / 3.0) * mu * (-(2.0 * Ux[0] * v1) + 2.0 * Ux[1] + Uy[0] * v2 - Uy[2]) / rho;
+            K1[2] = mu * (-(Ux[0] * v2) + Ux[2] - Uy[0] * v1 + Uy[1]) / rho;
+            K1[3] = (1.0 / 3.0) *
+                    (Cv * mu * (-(Uy[0] * v1 * v2) + 3.0 * Uy[1] * v2 - 2.0 * Uy[2] * v1) -
+                     Ux[0] * (Cv * mu * (3.0 * V_sq + (v1 * v1)) - 3.0 * lambda * (V_sq - en)) +
+                     Ux[1] * v1 * (4.0 * Cv * mu - 3.0 * lambda) + 3.0 * Ux[2] * v2 * (Cv * mu - lambda) +
+                     3.0 * Ux[3] * lambda) /
+                    (Cv * rho);
+            K2[1] = mu * (-(Ux[0] * v2) + Ux[2] - Uy[0] * v1 + Uy[1]) / rho;
+            K2[2] = (2.0 / 3.0) * mu * (Ux[0] * v1 - Ux[1] - 2.0 * Uy[0] * v2 + 2.0 * Uy[2]) / rho;
+            K2[3] = (1.0 / 3.0) *
+                    (Cv * mu * (-(Ux[0] * v1 * v2) - 2.0 * Ux[1] * v2 + 3.0 * Ux[2] * v1) -
+                     Uy[0] * (Cv * mu * (3.0 * V_sq + (v2 * v2)) - 3.0 * lambda * (V_sq - en)) +
+                     3.0 * Uy[1] * v1 * (Cv * mu - lambda) + Uy[2] * v2 * (4.0 * Cv * mu - 3.0 * lambda) +
+                     3.0 * Uy[3] * lambda) /
+                    (Cv * rho);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int q = 1; q < 4; ++q) rt[j][q] = rt[j][q] + (Nx[j] * K1[q] + Ny[j] * K2[q]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rt[j][q] = rt[j][q] * area * dtl / 3.0;
+}
+
+// FUENTE for one element (subrutinas.f90:1064-1078)
+__device__ inline void fuente_element(const double (&Uv)[3][4], const double (&Nx)[3],
+                                      const double (&Ny)[3], const double (&w_x)[3],
+                                      const double (&w_y)[3], double area, double dtl,
+                                      double (&rf)[3][4]) {
+    // sp(j,k): weight of local node j at quadrature point k
+    constexpr double sp[3][3] = {{0.5, 0.0, 0.5}, {0.5, 0.5, 0.0}, {0.0, 0.5, 0.5}};
+    double Ux[4], Uy[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        Ux[q] = Uv[0][q] * Nx[0] + Uv[1][q] * Nx[1] + Uv[2][q] * Nx[2];
+        Uy[q] = Uv[0][q] * Ny[0] + Uv[1][q] * Ny[1] + Uv[2][q] * Ny[2];
+    }
+    double AR = area * dtl / 3.0;
+    double wx[3], wy[3];
+#pragma unroll
+    for (int kq = 0; kq < 3; ++kq) {
+        double sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            sx = sx + sp[j][kq] * w_x[j];
+            sy = sy + sp[j][kq] * w_y[j];
+        }
+        wx[kq] = sx;
+        wy[kq] = sy;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            rf[j][q] = -(AR * (sp[j][0] * (Ux[q] * wx[0] + Uy[q] * wy[0]) +
+                               sp[j][1] * (Ux[q] * wx[1] + Uy[q] * wy[1]) +
+                               sp[j][2] * (Ux[q] * wx[2] + Uy[q] * wy[2])));
+}
+
+// One thread per element.  Connectivity, slot positions and the shape
+// gradients of the block's elements are staged through LDS with coalesced
+// loads; U/T of the three nodes are gathered (32 B + 8 B per node).
+template <bool kCalc, bool kTheta, bool kFuente>
+__global__ void __launch_bounds__(kBlock)
+k_element_rhs(ElemConsts kc, const int* __restrict__ inpoel, const int* __restrict__ epos,
+              const double* __restrict__ U, const double* __restrict__ theta,
+              const double* __restrict__ T, const double* __restrict__ dNx,
+              const double* __restrict__ dNy, const double* __restrict__ area,
+              const double* __restrict__ shoc, const double* __restrict__ dtl,
+              const double* __restrict__ t1, const double* __restrict__ t2,
+              const double* __restrict__ t3, const double* __restrict__ WX,
+              const double* __restrict__ WY, double* __restrict__ c_calc,
+              double* __restrict__ c_fuente, int nelem) {
+    __shared__ int s_in[3 * kBlock];
+    __shared__ int s_pos[3 * kBlock];
+    __shared__ double s_nx[3 * kBlock];
+    __shared__ double s_ny[3 * kBlock];
+    const int e0 = blockIdx.x * kBlock;
+    const int nloc = min(kBlock, nelem - e0);
+    for (int q = threadIdx.x; q < 3 * nloc; q += kBlock) {
+        s_in[q] = inpoel[3 * (long long)e0 + q];
+        s_pos[q] = epos[3 * (long long)e0 + q];
+        s_nx[q] = dNx[3 * (long long)e0 + q];
+        s_ny[q] = dNy[3 * (long long)e0 + q];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x >= nloc) return;
+    const int e = e0 + threadIdx.x;
+    const int l = 3 * threadIdx.x;
+    int p[3] = {s_in[l] - 1, s_in[l + 1] - 1, s_in[l + 2] - 1};
+    double Nx[3] = {s_nx[l], s_nx[l + 1], s_nx[l + 2]};
+    double Ny[3] = {s_ny[l], s_ny[l + 1], s_ny[l + 2]};
+    double Uv[3][4], Th[3][4], Tn[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        double4 u = ld4(U + 4 * (long long)p[j]);
+        Uv[j][0] = u.x; Uv[j][1] = u.y; Uv[j][2] = u.z; Uv[j][3] = u.w;
+        Tn[j] = kCalc ? T[p[j]] : 0.0;
+        if (kTheta) {
+            double4 t = ld4(theta + 4 * (long long)p[j]);
+            Th[j][0] = t.x; Th[j][1] = t.y; Th[j][2] = t.z; Th[j][3] = t.w;
+        } else {
+            Th[j][0] = Th[j][1] = Th[j][2] = Th[j][3] = 0.0;
+        }
+    }
+    const double ar = area[e], dt = dtl[e];
+    if (kCalc) {
+        double rt[3][4];
+        supg_element<kTheta>(kc, Uv, Th, Tn, Nx, Ny, t1[e], t2[e], t3[e], shoc[e], ar, dt, rt);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            st4(c_calc + 4 * (long long)s_pos[l + j], rt[j][0], rt[j][1], rt[j][2], rt[j][3]);
+    }
+    if (kFuente) {
+        double w_x[3] = {WX[p[0]], WX[p[1]], WX[p[2]]};
+        double w_y[3] = {WY[p[0]], WY[p[1]], WY[p[2]]};
+        double rf[3][4];
+        fuente_element(Uv, Nx, Ny, w_x, w_y, ar, dt, rf);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            st4(c_fuente + 4 * (long long)s_pos[l + j], rf[j][0], rf[j][1], rf[j][2], rf[j][3]);
+    }
+}
+
+// rhs_out(i) = rhs_in(i) + calc slots in order (+ FUENTE slots in order)
+__global__ void __launch_bounds__(kBlock)
+k_gather_rhs(const int* __restrict__ esup2, const double* rhs_in,
+             const double* __restrict__ c_calc, const double* __restrict__ c_fuente,
+             double* rhs_out, int npoin) {  // rhs_in may alias rhs_out (same index only)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin) return;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    if (rhs_in) {
+        double4 v = ld4(rhs_in + 4 * (long long)i);
+        r0 = v.x; r1 = v.y; r2 = v.z; r3 = v.w;
+    }
+    const int b = esup2[i], e = esup2[i + 1];
+    for (int s = b; s < e; ++s) {
+        double4 c = ld4(c_calc + 4 * (long long)s);
+        r0 = r0 + c.x; r1 = r1 + c.y; r2 = r2 + c.z; r3 = r3 + c.w;
+    }
+    if (c_fuente)
+        for (int s = b; s < e; ++s) {
+            double4 c = ld4(c_fuente + 4 * (long long)s);
+            r0 = r0 + c.x; r1 = r1 + c.y; r2 = r2 + c.z; r3 = r3 + c.w;
+        }
+    st4(rhs_out + 4 * (long long)i, r0, r1, r2, r3);
+}
+
+// Gather + one RK stage for one node (subrutinas.f90:679-826, NGAS = 0)
+template <bool kFuente>
+__global__ void __launch_bounds__(kBlock)
+k_stage(const int* __restrict__ esup2, const double* __restrict__ c_calc,
+        const double* __restrict__ c_fuente, double rk_fact, double fr,
+        const double* __restrict__ U, const double* __restrict__ M, const double* __restrict__ GAMM,
+        const double* __restrict__ WX, const double* __restrict__ WY,
+        const int* __restrict__ bcidx, const int* __restrict__ bcflags,
+        const double* __restrict__ bcvals, double* __restrict__ U1, double* __restrict__ RHS,
+        double* __restrict__ T, double* __restrict__ VX, double* __restrict__ VY,
+        double* __restrict__ RHO, double* __restrict__ E, double* __restrict__ P,
+        double* __restrict__ RMACH, int npoin) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin) return;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    const int b = esup2[i], e = esup2[i + 1];
+    for (int s = b; s < e; ++s) {
+        double4 c = ld4(c_calc + 4 * (long long)s);
+        r0 = r0 + c.x; r1 = r1 + c.y; r2 = r2 + c.z; r3 = r3 + c.w;
+    }
+    if (kFuente)
+        for (int s = b; s < e; ++s) {
+            double4 c = ld4(c_fuente + 4 * (long long)s);
+            r0 = r0 + c.x; r1 = r1 + c.y; r2 = r2 + c.z; r3 = r3 + c.w;
+        }
+    if (RHS) st4(RHS + 4 * (long long)i, r0, r1, r2, r3);
+    double4 u = ld4(U + 4 * (long long)i);
+    double f = rk_fact / M[i];
+    double u0 = u.x - f * r0, u1 = u.y - f * r1, u2 = u.z - f * r2, u3 = u.w - f * r3;
+    double gamm = GAMM[i];
+    double rho = u0;
+    double vx = u1 / rho;
+    double vy = u2 / rho;
+    double en = u3 / rho;
+    double vel2 = vx * vx + vy * vy;
+    double pr = rho * (gamm - 1.0) * (en - 0.5 * vel2);
+    double tt = pr / (rho * fr);
+    P[i] = pr;
+    RMACH[i] = sqrt(vel2 / (tt * gamm * fr));
+    int bi = bcidx[i];
+    if (bi >= 0) {
+        int fl = bcflags[bi];
+        const double* v = bcvals + 8 * bi;
+        if (fl & 1) { vx = v[0]; vy = v[1]; }                 // FIXVEL
+        if (fl & 2) {                                          // NORMALVEL
+            double wx = WX[i], wy = WY[i], nx = v[2], ny = v[3];
+            double pp = -ny * (vx - wx) + nx * (vy - wy);
+            vx = -ny * pp + wx;
+            vy = nx * pp + wy;
+        }
+        if (fl & 4) rho = v[4];                                // FIX rho
+        if (fl & 8) {                                          // FIX T
+            double gm = gamm - 1.0;
+            tt = v[5];
+            en = tt * fr / gm + 0.5 * (vx * vx + vy * vy);
+        }
+    }
+    RHO[i] = rho;
+    VX[i] = vx;
+    VY[i] = vy;
+    E[i] = en;
+    T[i] = tt;
+    st4(U1 + 4 * (long long)i, rho, vx * rho, vy * rho, en * rho);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void launch_deltat(const Tables& t, const Consts& c, const double* area, const double* T,
+                   const double* VX, const double* VY, const double* WX, const double* WY,
+                   double* DT, double* dtmin_out, double* partials, unsigned* counter,
+                   hipStream_t s) {
+    k_deltat<<<red_grid(t.nelem), kBlock, 0, s>>>(t.inpoel.p, area, T, VX, VY, WX, WY, DT, t.nelem,
+                                                  c.t_inf, c.fsafe, dtmin_out, partials, counter);
+    k_deltat_clip<<<grid_for(t.nelem), kBlock, 0, s>>>(DT, dtmin_out, t.nelem);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_estab(const Tables& t, const Consts& c, const double* U, const double* T,
+                  const double* GAMM, const double* VX, const double* VY, const double* WX,
+                  const double* WY, const double* dNx, const double* dNy, double dtmin,
+                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s) {
+    k_estab<<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy,
+                                                 t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin, shoc, t1,
+                                                 t2, t3);
+    CFD_HIP(hipGetLastError());
+}
+
+static ElemConsts elem_consts(const Consts& c) {
+    ElemConsts k;
+    k.g = c.gama;
+    k.Cv = c.fcv;
+    k.lambda_ref = c.fk;
+    k.mu_ref = c.fmu;
+    k.T_inf = c.t_inf;
+    k.cte = c.cte;
+    k.viscous = c.fmu > 2.2250738585072014e-308 ? 1 : 0;  // mu_ref > tiny(0d0)
+    return k;
+}
+
+void launch_element_rhs(const Tables& t, const Consts& c, const ElemIn& in, double* c_calc,
+                        double* c_fuente, hipStream_t s) {
+    ElemConsts k = elem_consts(c);
+    dim3 g(grid_for(t.nelem)), b(kBlock);
+    const bool th = in.theta != nullptr, fu = in.WX != nullptr;
+#define CFD_ELEM(TH, FU)                                                                      \
+    k_element_rhs<true, TH, FU><<<g, b, 0, s>>>(k, t.inpoel.p, t.epos.p, in.U, in.theta, in.T,       \
+                                          in.dNx, in.dNy, in.area, in.shoc, in.dtl, in.t1,     \
+                                          in.t2, in.t3, in.WX, in.WY, c_calc, c_fuente, t.nelem)
+    if (th && fu) CFD_ELEM(true, true);
+    else if (th) CFD_ELEM(true, false);
+    else if (fu) CFD_ELEM(false, true);
+    else CFD_ELEM(false, false);
+#undef CFD_ELEM
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_fuente(const Tables& t, const ElemIn& in, double* c_fuente, hipStream_t s) {
+    ElemConsts k{};
+    k_element_rhs<false, false, true><<<grid_for(t.nelem), kBlock, 0, s>>>(
+        k, t.inpoel.p, t.epos.p, in.U, nullptr, nullptr, in.dNx, in.dNy, in.area, nullptr, in.dtl,
+        nullptr, nullptr, nullptr, in.WX, in.WY, nullptr, c_fuente, t.nelem);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_gather_rhs(const Tables& t, const double* rhs_in, const double* c_calc,
+                       const double* c_fuente, double* rhs_out, hipStream_t s) {
+    k_gather_rhs<<<grid_for(t.npoin), kBlock, 0, s>>>(t.esup2.p, rhs_in, c_calc, c_fuente, rhs_out,
+                                                      t.npoin);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
+                         const double* c_calc, const double* c_fuente, const StageNodes& nd,
+                         bool write_rhs, hipStream_t s) {
+    dim3 g(grid_for(t.npoin)), b(kBlock);
+    double* rhs = write_rhs ? nd.RHS : nullptr;
+    if (c_fuente)
+        k_stage<true><<<g, b, 0, s>>>(t.esup2.p, c_calc, c_fuente, rk_fact, c.fr, nd.U, nd.M, nd.GAMM,
+                                      nd.WX, nd.WY, bc.bcidx.p, bc.flags.p, bc.vals.p, nd.U1, rhs,
+                                      nd.T, nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH, t.npoin);
+    else
+        k_stage<false><<<g, b, 0, s>>>(t.esup2.p, c_calc, c_fuente, rk_fact, c.fr, nd.U, nd.M,
+                                       nd.GAMM, nd.WX, nd.WY, bc.bcidx.p, bc.flags.p, bc.vals.p,
+                                       nd.U1, rhs, nd.T, nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH,
+                                       t.npoin);
+    CFD_HIP(hipGetLastError());
+}
+
+}  // namespace cfd

@@ -1,0 +1,272 @@
+// Geometry kernels of the path: P1 gradients and areas (deriv), lumped mass
+// (MASAS), mesh-Laplace values (mLaplace), the GCL mass update (gcl.f90),
+// wall normals (NORMALES), and the small per-set loops of fluidStructure
+// (FORCES, TRANSF).
+//
+// Node-owned results are gathers over each node's esup segment in ascending
+// element order: that is the order in which the reference's sequential
+// element-order scatter adds into the node, so sums are bitwise identical and
+// no atomics are needed.  All of these are HBM-bound integer/gather work.
+#include "cfd_internal.hpp"
+#include "reduce.hpp"
+
+namespace cfd {
+namespace {
+
+// subrutinas.f90:99-125
+__global__ void __launch_bounds__(kBlock)
+k_deriv(const int* __restrict__ inpoel, const double* __restrict__ X, const double* __restrict__ Y,
+        double* __restrict__ area, double* __restrict__ dNx, double* __restrict__ dNy,
+        double* __restrict__ HH, double* __restrict__ HHX, double* __restrict__ HHY, int nelem,
+        double* __restrict__ hmin_out, double* __restrict__ partials, unsigned* __restrict__ counter) {
+    double hm = __builtin_huge_val();
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += gridDim.x * blockDim.x) {
+        int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
+        double X1 = X[n1], X2 = X[n2], X3 = X[n3];
+        double Y1 = Y[n1], Y2 = Y[n2], Y3 = Y[n3];
+        double a = (X2 * Y3 + X3 * Y1 + X1 * Y2 - (X2 * Y1 + X3 * Y2 + X1 * Y3)) / 2.0;
+        area[e] = a;
+        double a2 = 2.0 * a;
+        dNx[3 * e + 0] = (Y2 - Y3) / a2;
+        dNx[3 * e + 1] = (Y3 - Y1) / a2;
+        dNx[3 * e + 2] = (Y1 - Y2) / a2;
+        dNy[3 * e + 0] = (X3 - X2) / a2;
+        dNy[3 * e + 1] = (X1 - X3) / a2;
+        dNy[3 * e + 2] = (X2 - X1) / a2;
+        double h = sqrt(a);
+        HH[e] = h;
+        if (h < hm) hm = h;
+        double mx = X3 - X2, my = Y3 - Y2;
+        if (X1 - X3 < mx) mx = X1 - X3;
+        if (X2 - X1 < mx) mx = X2 - X1;
+        if (Y1 - Y3 < my) my = Y1 - Y3;
+        if (Y2 - Y1 < my) my = Y2 - Y1;
+        HHX[e] = fabs(mx);
+        HHY[e] = fabs(my);
+    }
+    if (block_partial(hm, OpMin(), partials, counter)) {
+        double m = combine_partials(partials, gridDim.x, __builtin_huge_val(), OpMin());
+        if (threadIdx.x == 0) {
+            if (m > 1.0e10) m = 1.0e10;
+            *hmin_out = m;
+            *counter = 0;
+        }
+    }
+}
+
+// subrutinas.f90:137-152: M(i) = sum of area/3 over i's elements, ascending
+__global__ void __launch_bounds__(kBlock)
+k_masas(const int* __restrict__ esup1, const int* __restrict__ esup2, const double* __restrict__ area,
+        double* __restrict__ M, int npoin) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin) return;
+    double m = 0.0;
+    for (int s = esup2[i]; s < esup2[i + 1]; ++s) m = m + area[esup1[s] - 1] / 3.0;
+    M[i] = m;
+}
+
+// mLaplace.f90:188-200
+__device__ inline double quality_mu(double X1, double X2, double X3, double Y1, double Y2, double Y3) {
+    const double TWOSQRT3 = 3.46410161513775;
+    double ar = X2 * Y3 + X3 * Y1 + X1 * Y2 - (X2 * Y1 + X3 * Y2 + X1 * Y3);
+    double l1 = (X3 - X2) * (X3 - X2) + (Y3 - Y2) * (Y3 - Y2);
+    double l2 = (X1 - X3) * (X1 - X3) + (Y1 - Y3) * (Y1 - Y3);
+    double l3 = (X2 - X1) * (X2 - X1) + (Y2 - Y1) * (Y2 - Y1);
+    double l = l1 + l2 + l3;
+    return TWOSQRT3 * ar / l;
+}
+
+// mLaplace.f90:120-148, one row per thread; column positions from the slot map
+__global__ void __launch_bounds__(kBlock)
+k_laplace(const int* __restrict__ inpoel, const int* __restrict__ esup1, const int* __restrict__ esup2,
+          const int* __restrict__ rowptr, const int* __restrict__ lslot, const double* __restrict__ X,
+          const double* __restrict__ Y, const double* __restrict__ dNx, const double* __restrict__ dNy,
+          double* __restrict__ lap, double* __restrict__ diag, int npoin) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin) return;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) lap[k] = 0.0;
+    for (int s = esup2[i]; s < esup2[i + 1]; ++s) {
+        int e = esup1[s] - 1;
+        int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
+        double mu = quality_mu(X[n1], X[n2], X[n3], Y[n1], Y[n2], Y[n3]);
+        double q = 1.0 / (mu * mu);
+        int li = (n1 == i) ? 0 : ((n2 == i) ? 1 : 2);
+        double ax = dNx[3 * e + li], ay = dNy[3 * e + li];
+        for (int j = 0; j < 3; ++j) {
+            int k = lslot[3 * s + j];
+            lap[k] = lap[k] + (ax * dNx[3 * e + j] + ay * dNy[3 * e + j]) * q;
+        }
+    }
+    diag[i] = lap[rowptr[i]];
+}
+
+// gcl.f90:218-236 per node: both divergence sums in ascending element order
+__global__ void __launch_bounds__(kBlock)
+k_gcl(const int* __restrict__ inpoel, const int* __restrict__ esup1, const int* __restrict__ esup2,
+      double* __restrict__ M, const double* __restrict__ Wx, const double* __restrict__ Wy,
+      const double* __restrict__ Wxo, const double* __restrict__ Wyo, const double* __restrict__ dNx,
+      const double* __restrict__ dNy, const double* __restrict__ area,
+      const double* __restrict__ area_old, double dt, int mode, int npoin) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin) return;
+    const double* wy = mode ? Wy : Wx;     // as written, the y-part reads W_x
+    const double* wyo = mode ? Wyo : Wxo;
+    double tot1 = 0.0, tot2 = 0.0;
+    for (int s = esup2[i]; s < esup2[i + 1]; ++s) {
+        int e = esup1[s] - 1;
+        int n0 = inpoel[3 * e] - 1, n1 = inpoel[3 * e + 1] - 1, n2 = inpoel[3 * e + 2] - 1;
+        const double* nx = dNx + 3 * e;
+        const double* ny = dNy + 3 * e;
+        double divW = nx[0] * Wx[n0] + nx[1] * Wx[n1] + nx[2] * Wx[n2] + ny[0] * wy[n0] +
+                      ny[1] * wy[n1] + ny[2] * wy[n2];
+        double divWo = nx[0] * Wxo[n0] + nx[1] * Wxo[n1] + nx[2] * Wxo[n2] + ny[0] * wyo[n0] +
+                       ny[1] * wyo[n1] + ny[2] * wyo[n2];
+        tot1 = tot1 + divW * area[e] / 3.0;
+        tot2 = tot2 + divWo * area_old[e] / 3.0;
+    }
+    M[i] = M[i] + dt * (tot1 + tot2) / 2.0;
+}
+
+// subrutinas.f90:26-63 per wall node; incident wall edges in ascending order
+__global__ void k_normales(int nw, const int* __restrict__ wnode_bc, const int* __restrict__ wnode_ptr,
+                           const int* __restrict__ wnode_edges, const int* __restrict__ wall,
+                           const double* __restrict__ X, const double* __restrict__ Y,
+                           int* __restrict__ flags, double* __restrict__ vals) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    double nlx = 0.0, nly = 0.0, den = 0.0;
+    for (int q = wnode_ptr[w]; q < wnode_ptr[w + 1]; ++q) {
+        int ed = wnode_edges[q];
+        int a = wall[2 * ed] - 1, b = wall[2 * ed + 1] - 1;
+        double lx = Y[b] - Y[a];
+        double ly = -(X[b] - X[a]);
+        double l = sqrt(lx * lx + ly * ly);
+        nlx = nlx + lx;
+        nly = nly + ly;
+        den = den + l;
+    }
+    int bi = wnode_bc[w];
+    int f = flags[bi] & ~2;
+    if (den > 1.0e-6) {
+        double lx = nlx / den, ly = nly / den;
+        double ln = sqrt(lx * lx + ly * ly);
+        if (ln > 0.2) {
+            f |= 2;
+            vals[8 * bi + 2] = lx / ln;
+            vals[8 * bi + 3] = ly / ln;
+        }
+    }
+    flags[bi] = f;
+}
+
+// meshMove.f90:167-192, one set; per-edge terms in parallel, summed in edge order
+__global__ void __launch_bounds__(1024)
+k_forces(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
+         const double* __restrict__ Y, const double* __restrict__ P, double xref, double yref,
+         double* __restrict__ out) {
+    __shared__ double dfx[1024], dfy[1024], m1[1024], m2[1024];
+    double FX = 0.0, FY = 0.0, RM = 0.0;
+    for (int base = 0; base < nedge; base += 1024) {
+        int i = base + threadIdx.x;
+        if (i < nedge) {
+            int n1 = iset[3 * i] - 1, n2 = iset[3 * i + 1] - 1;
+            double d_press = (P[n1] + P[n2]) / 2.0;
+            double rlx = X[n1] - X[n2];
+            double rly = Y[n2] - Y[n1];
+            double fx = d_press * rly, fy = d_press * rlx;
+            double xc = (X[n1] + X[n2]) / 2.0;
+            double yc = (Y[n2] + Y[n1]) / 2.0;
+            dfx[threadIdx.x] = fx;
+            dfy[threadIdx.x] = fy;
+            m1[threadIdx.x] = fy * (xc - xref);
+            m2[threadIdx.x] = fx * (yc - yref);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int m = nedge - base < 1024 ? nedge - base : 1024;
+            for (int q = 0; q < m; ++q) {   // RM = RM + DFY*(XC-XR) - DFX*(YC-YR)
+                FX = FX + dfx[q];
+                FY = FY + dfy[q];
+                RM = RM + m1[q] - m2[q];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = FX;
+        out[1] = FY;
+        out[2] = RM;
+    }
+}
+
+// meshMove.f90:378-389, one set
+__global__ void k_transf(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
+                         const double* __restrict__ Y, double ca, double sa, double yposr,
+                         double xref, double yref, double* __restrict__ DXPOS,
+                         double* __restrict__ DYPOS) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nedge) return;
+    for (int jj = 0; jj < 2; ++jj) {
+        int n = iset[3 * i + jj] - 1;
+        double distx = X[n] - xref;
+        double disty = Y[n] - yref + yposr;
+        DXPOS[n] = ca * distx + sa * disty - distx;
+        DYPOS[n] = -(sa * distx) + ca * disty - disty + yposr;
+    }
+}
+
+}  // namespace
+
+void launch_deriv(const Tables& t, const double* X, const double* Y, double* area, double* dNx,
+                  double* dNy, double* HH, double* HHX, double* HHY, double* hmin_out,
+                  double* partials, unsigned* counter, hipStream_t s) {
+    k_deriv<<<red_grid(t.nelem), kBlock, 0, s>>>(t.inpoel.p, X, Y, area, dNx, dNy, HH, HHX, HHY,
+                                                 t.nelem, hmin_out, partials, counter);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_masas(const Tables& t, const double* area, double* M, hipStream_t s) {
+    k_masas<<<grid_for(t.npoin), kBlock, 0, s>>>(t.esup1.p, t.esup2.p, area, M, t.npoin);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_laplace(const Tables& t, const double* X, const double* Y, const double* dNx,
+                    const double* dNy, double* lap, double* diag, hipStream_t s) {
+    k_laplace<<<grid_for(t.npoin), kBlock, 0, s>>>(t.inpoel.p, t.esup1.p, t.esup2.p, t.rowptr.p,
+                                                   t.lslot.p, X, Y, dNx, dNy, lap, diag, t.npoin);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_gcl(const Tables& t, double* M, const double* Wx, const double* Wy, const double* Wxo,
+                const double* Wyo, const double* dNx, const double* dNy, const double* area,
+                const double* area_old, double dt, int mode, hipStream_t s) {
+    k_gcl<<<grid_for(t.npoin), kBlock, 0, s>>>(t.inpoel.p, t.esup1.p, t.esup2.p, M, Wx, Wy, Wxo, Wyo,
+                                               dNx, dNy, area, area_old, dt, mode, t.npoin);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_normales(const BcTable& bc, int nw, const int* wnode_bc, const int* wnode_ptr,
+                     const int* wnode_edges, const int* wall, const double* X, const double* Y,
+                     hipStream_t s) {
+    if (nw == 0) return;
+    k_normales<<<grid_for(nw), kBlock, 0, s>>>(nw, wnode_bc, wnode_ptr, wnode_edges, wall, X, Y,
+                                               bc.flags.p, bc.vals.p);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_forces(int nedge, const int* iset, const double* X, const double* Y, const double* P,
+                   double xref, double yref, double* out3, hipStream_t s) {
+    k_forces<<<1, 1024, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_transf(int nedge, const int* iset, const double* X, const double* Y, double ca,
+                   double sa, double yposr, double xref, double yref, double* DXPOS,
+                   double* DYPOS, hipStream_t s) {
+    if (nedge == 0) return;
+    k_transf<<<grid_for(nedge), kBlock, 0, s>>>(nedge, iset, X, Y, ca, sa, yposr, xref, yref, DXPOS,
+                                                DYPOS);
+    CFD_HIP(hipGetLastError());
+}
+
+}  // namespace cfd

@@ -1,0 +1,250 @@
+// Jacobi-preconditioned CG of biconjGrad.f90:8-62 (named biCG there).
+//
+// Reference sequence (one solve):
+//   x(fix) = x_fix; y = A x; y(fix) = 1e30 x(fix); r = b - y; r(fix) = 0
+//   if r.r < tol: return
+//   p = r/diag; e = r.p; y = A p (fix rows 1e30 p); alfa = e/p.y; x += alfa p
+//   while |e_old| > tol and k < 1000:
+//     k++; r -= alfa y; z = r/diag; e = r.z; beta = e/e_old; p = beta p + z
+//     y = A p (fix rows 1e30 p); alfa = e/p.y; x += alfa p; e_old = e
+//
+// On the device one iteration is two kernels:
+//   A: x += alfa p (the previous iteration's update), r -= alfa y, z = r/diag,
+//      block partials of r.z; the last block forms e and beta.
+//   B: each row forms p_new = beta p_old + z for its columns on the fly, y = A
+//      p_new, stores its own p_new (double-buffered p), block partials of p.y;
+//      the last block forms alfa and the loop test.
+// Scalars live in device memory, so the host only polls a done flag every
+// few iterations.  The kernels run a fixed grid-stride grid, and dot products
+// combine block partials in a fixed tree, so the iteration sequence is
+// deterministic (not the serial summation order of the reference).
+#include "cfd_internal.hpp"
+#include "reduce.hpp"
+
+namespace cfd {
+namespace {
+
+// scal[] slots
+enum { S_ALFA = 0, S_BETA, S_EOLD, S_ENEW, S_PY, S_RR, S_K, S_DONE, S_STOP, S_CONV, S_TOL, S_MAXIT };
+
+#define GRID_STRIDE(i, n) \
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+__global__ void k_fix_init(int n, int* __restrict__ fixflag) {
+    GRID_STRIDE(i, n) fixflag[i] = 0;
+}
+
+__global__ void k_fix_set(int nfix, const int* __restrict__ idx, const double* __restrict__ x_fix,
+                          int* __restrict__ fixflag, double* __restrict__ x) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nfix) return;
+    int i = idx[q] - 1;
+    fixflag[i] = 1;
+    x[i] = 1.0 * x_fix[q];
+}
+
+__global__ void k_scal_init(double* __restrict__ scal, double tol, int maxit) {
+    for (int q = 0; q < 16; ++q) scal[q] = 0.0;
+    scal[S_TOL] = tol;
+    scal[S_MAXIT] = (double)maxit;
+}
+
+// r = -1*(A x) + b with fixed rows y = 1e30 x and r(fix) = 0; partial r.r
+__global__ void __launch_bounds__(kBlock)
+k_residual(int n, const double* __restrict__ A, const int* __restrict__ idx,
+           const int* __restrict__ rowptr, const int* __restrict__ fixflag,
+           const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
+           double* __restrict__ scal, double* __restrict__ partials, unsigned* __restrict__ counter) {
+    double v = 0.0;
+    GRID_STRIDE(i, n) {
+        double ri;
+        if (fixflag[i]) {
+            ri = 0.0;
+        } else {
+            double dot = 0.0;
+            for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) dot = dot + A[j] * x[idx[j] - 1];
+            ri = -1.0 * dot + b[i];
+        }
+        r[i] = ri;
+        v = v + ri * ri;
+    }
+    if (!block_partial(v, OpSum(), partials, counter)) return;
+    double rr = combine_partials(partials, gridDim.x, 0.0, OpSum());
+    if (threadIdx.x == 0) {
+        scal[S_RR] = rr;
+        if (rr < scal[S_TOL]) {
+            scal[S_DONE] = 1.0;
+            scal[S_CONV] = 1.0;
+            scal[S_EOLD] = rr;
+        }
+        *counter = 0;
+    }
+}
+
+// first preconditioned residual: z = r/diag, e = r.z
+__global__ void __launch_bounds__(kBlock)
+k_first_z(int n, const double* __restrict__ r, const double* __restrict__ diag,
+          double* __restrict__ z, double* __restrict__ scal, double* __restrict__ partials,
+          unsigned* __restrict__ counter) {
+    if (scal[S_DONE] != 0.0) return;
+    double v = 0.0;
+    GRID_STRIDE(i, n) {
+        double zi = r[i] / diag[i];
+        z[i] = zi;
+        v = v + r[i] * zi;
+    }
+    if (!block_partial(v, OpSum(), partials, counter)) return;
+    double e = combine_partials(partials, gridDim.x, 0.0, OpSum());
+    if (threadIdx.x == 0) {
+        scal[S_ENEW] = e;
+        *counter = 0;
+    }
+}
+
+// Kernel A of an iteration (see the file header)
+__global__ void __launch_bounds__(kBlock)
+k_iter_a(int n, double* __restrict__ x, const double* __restrict__ pcur, double* __restrict__ r,
+         const double* __restrict__ y, const double* __restrict__ diag, double* __restrict__ z,
+         double* __restrict__ scal, double* __restrict__ partials, unsigned* __restrict__ counter) {
+    if (scal[S_DONE] != 0.0) return;
+    const double alfa = scal[S_ALFA];
+    const bool stop = scal[S_STOP] != 0.0;
+    double v = 0.0;
+    GRID_STRIDE(i, n) {
+        x[i] = alfa * pcur[i] + x[i];
+        if (!stop) {
+            double ri = -alfa * y[i] + r[i];
+            r[i] = ri;
+            double zi = ri / diag[i];
+            z[i] = zi;
+            v = v + ri * zi;
+        }
+    }
+    if (!block_partial(v, OpSum(), partials, counter)) return;
+    double e = combine_partials(partials, gridDim.x, 0.0, OpSum());
+    if (threadIdx.x == 0) {
+        if (stop) {
+            scal[S_DONE] = 1.0;
+            double eo = scal[S_EOLD];
+            scal[S_CONV] = fabs(eo) > scal[S_TOL] ? 0.0 : 1.0;
+        } else {
+            scal[S_K] = scal[S_K] + 1.0;
+            scal[S_ENEW] = e;
+            scal[S_BETA] = e / scal[S_EOLD];
+        }
+        *counter = 0;
+    }
+}
+
+// Kernel B: p_new = beta p_old + z on the fly (first: p_new = z), y = A p_new
+__global__ void __launch_bounds__(kBlock)
+k_iter_b(int n, const double* __restrict__ A, const int* __restrict__ idx,
+         const int* __restrict__ rowptr, const int* __restrict__ fixflag,
+         const double* __restrict__ pold, const double* __restrict__ z, double* __restrict__ pnew,
+         double* __restrict__ y, int first, double* __restrict__ scal,
+         double* __restrict__ partials, unsigned* __restrict__ counter) {
+    if (scal[S_DONE] != 0.0 || scal[S_STOP] != 0.0) return;
+    const double beta = scal[S_BETA];
+    double v = 0.0;
+    GRID_STRIDE(i, n) {
+        double pi = first ? z[i] : beta * pold[i] + z[i];
+        double yi;
+        if (fixflag[i]) {
+            yi = 1.0e30 * pi;
+        } else {
+            double dot = 0.0;
+            for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) {
+                int c = idx[j] - 1;
+                double pc = first ? z[c] : beta * pold[c] + z[c];
+                dot = dot + A[j] * pc;
+            }
+            yi = dot;
+        }
+        pnew[i] = pi;
+        y[i] = yi;
+        v = v + pi * yi;
+    }
+    if (!block_partial(v, OpSum(), partials, counter)) return;
+    double py = combine_partials(partials, gridDim.x, 0.0, OpSum());
+    if (threadIdx.x == 0) {
+        double e = scal[S_ENEW];
+        scal[S_PY] = py;
+        scal[S_ALFA] = e / py;
+        scal[S_EOLD] = e;
+        bool more = fabs(e) > scal[S_TOL] && scal[S_K] < scal[S_MAXIT];
+        if (!more) scal[S_STOP] = 1.0;
+        *counter = 0;
+    }
+}
+
+}  // namespace
+
+void PcgWork::ensure(int npoin) {
+    if (npoin == n && r.p) return;
+    n = npoin;
+    nblocks = (int)red_grid(npoin);
+    r.alloc(npoin);
+    z.alloc(npoin);
+    y.alloc(npoin);
+    pa.alloc(npoin);
+    pb.alloc(npoin);
+    fixflag.alloc(npoin);
+    partial.alloc(nblocks);
+    scal.alloc(16);
+    counter.alloc(1);
+    CFD_HIP(hipMemset(counter.p, 0, sizeof(unsigned)));
+}
+
+PcgResult pcg_solve(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
+                    const double* diag, double* x, const double* b, const double* x_fix,
+                    const int* x_fix_idx, int nfix, double tol, int maxit, int check_every,
+                    hipStream_t s) {
+    w.ensure(npoin);
+    const dim3 g(red_grid(npoin)), blk(kBlock);
+    double* scal = w.scal.p;
+    double h[16];
+    auto poll = [&]() {
+        CFD_HIP(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, s));
+        CFD_HIP(hipStreamSynchronize(s));
+    };
+    k_scal_init<<<1, 1, 0, s>>>(scal, tol, maxit);
+    k_fix_init<<<g, blk, 0, s>>>(npoin, w.fixflag.p);
+    if (nfix > 0)
+        k_fix_set<<<grid_for(nfix), blk, 0, s>>>(nfix, x_fix_idx, x_fix, w.fixflag.p, x);
+    k_residual<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, x, b, w.r.p, scal,
+                                 w.partial.p, w.counter.p);
+    k_first_z<<<g, blk, 0, s>>>(npoin, w.r.p, diag, w.z.p, scal, w.partial.p, w.counter.p);
+    double* pbuf[2] = {w.pa.p, w.pb.p};
+    k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[1], w.z.p, pbuf[0], w.y.p, 1,
+                               scal, w.partial.p, w.counter.p);
+    CFD_HIP(hipGetLastError());
+    poll();  // most warm-started mesh solves stop here (r.r < tol)
+    int cur = 0;  // pbuf[cur] holds the current p
+    const int chunk = check_every > 0 ? check_every : 8;
+    for (int launched = 0; h[S_DONE] == 0.0; launched += chunk) {
+        if (h[S_STOP] != 0.0) {
+            // the loop test failed: one A applies the final x update and ends the solve
+            k_iter_a<<<g, blk, 0, s>>>(npoin, x, pbuf[cur], w.r.p, w.y.p, diag, w.z.p, scal,
+                                       w.partial.p, w.counter.p);
+            poll();
+            break;
+        }
+        for (int q = 0; q < chunk; ++q) {
+            k_iter_a<<<g, blk, 0, s>>>(npoin, x, pbuf[cur], w.r.p, w.y.p, diag, w.z.p, scal,
+                                       w.partial.p, w.counter.p);
+            k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[cur], w.z.p,
+                                       pbuf[cur ^ 1], w.y.p, 0, scal, w.partial.p, w.counter.p);
+            cur ^= 1;
+        }
+        CFD_HIP(hipGetLastError());
+        poll();
+        CFD_CHECK(launched <= 4 * maxit + 64, "PCG did not terminate");
+    }
+    PcgResult res;
+    res.iters = (int)h[S_K];
+    res.rz = h[S_EOLD];
+    res.converged = h[S_CONV] != 0.0;
+    return res;
+}
+
+}  // namespace cfd

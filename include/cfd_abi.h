@@ -1,0 +1,217 @@
+/* cfd_abi.h -- C ABI of libcfd_amd.so, the gfx950 (MI355X) implementation of
+ * the chanshing/cfd SUPG/ALE hot path.
+ *
+ * Two layers:
+ *  1. Host-pointer entries, one per reference routine on the path, with the
+ *     reference's argument meaning and accumulate/overwrite semantics.  They
+ *     take plain host arrays, run the HIP kernels on the calling thread's
+ *     mesh context and return when the results are back in host memory.
+ *     A Fortran shim module with the reference module/procedure names
+ *     (cfd_amd/fortran/cfd_shims.f90) forwards to them via iso_c_binding.
+ *  2. A device-resident session that replays the main program's time step
+ *     (ns2DComp.ALE.f90:138-281) with all state in HBM.
+ *
+ * Conventions (the reference's): node and element ids stored in arrays are
+ * 1-based, offset arrays are 0-based with a leading 0, arrays are Fortran
+ * column-major: U(4,npoin) is 4 contiguous doubles per node, inpoel(3,nelem)
+ * and dNx(3,nelem) are 3 contiguous values per element.  Integers are int32,
+ * reals are binary64.
+ *
+ * Errors: every entry returns 0 on success and a non-zero code otherwise;
+ * cfd_last_error() returns the message of the last failure on this thread.
+ * The reference has no status codes (it STOPs or, for the PCG, silently
+ * returns after 1000 iterations); the shims turn non-zero codes into
+ * ERROR STOP.  There is no CPU fallback: without a usable GPU every entry
+ * fails with CFD_ERR_DEVICE.
+ */
+#ifndef CFD_ABI_H
+#define CFD_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_ABI_VERSION 1
+
+enum {
+    CFD_OK = 0,
+    CFD_ERR_ARG = 1,      /* invalid argument or size                     */
+    CFD_ERR_DEVICE = 2,   /* HIP runtime / device failure, no GPU          */
+    CFD_ERR_MESH = 3,     /* mesh rejected (ids out of range, degenerate)  */
+    CFD_ERR_STATE = 4     /* call out of sequence (e.g. GCL before putW)   */
+};
+
+typedef struct cfd_mesh cfd_mesh;        /* tables + scratch for one mesh  */
+typedef struct cfd_session cfd_session;  /* device-resident solver state   */
+
+/* InputData constants the path reads (dataLoader.f90:5-8, derived at :59-64).
+ * cte is the shock-capturing factor after the loader's CTE := 1/CTE. */
+typedef struct {
+    double gama, fr, fmu, fk, fcv;
+    double t_inf, rho_inf, u_inf, v_inf;
+    double cte, fsafe;
+} cfd_consts;
+
+int cfd_abi_version(void);
+const char *cfd_last_error(void);
+int cfd_device_count(int *count);
+
+/* ---- mesh context ------------------------------------------------------ */
+/* Builds esup, psup, the Laplace CSR structure and the gather slot maps on
+ * the device from inpoel(3,nelem) (host, 1-based). */
+int cfd_mesh_create(cfd_mesh **out, int nelem, int npoin, const int *inpoel);
+int cfd_mesh_destroy(cfd_mesh *m);
+/* sizes of psup1 and of lap_idx/lap_sparse */
+int cfd_mesh_sizes(const cfd_mesh *m, int *npsup, int *nnz);
+
+/* PointNeighbor::getEsup (pointNeighbor.f90:5-43): esup1(3*nelem), esup2(npoin+1) */
+int cfd_get_esup(cfd_mesh *m, int *esup1, int *esup2);
+/* PointNeighbor::getPsup (pointNeighbor.f90:45-91): psup1(npsup), psup2(npoin+1) */
+int cfd_get_psup(cfd_mesh *m, int *psup1, int *psup2);
+/* Mlaplace initialize (mLaplace.f90:60-94): lap_rowptr(npoin+1), lap_idx(nnz) */
+int cfd_lap_structure(cfd_mesh *m, int *lap_rowptr, int *lap_idx);
+
+/* Mlaplace::laplace (mLaplace.f90:7-58): values of the quality-weighted mesh
+ * Laplacian; X, Y are the module coordinates the reference reads by USE. */
+int cfd_laplace(cfd_mesh *m, const double *X, const double *Y, const double *dNx,
+                const double *dNy, double *lap_sparse, double *lap_diag);
+
+/* deriv (subrutinas.f90:88-126) */
+int cfd_deriv(cfd_mesh *m, const double *X, const double *Y, double *area, double *dNx,
+              double *dNy, double *HH, double *HHX, double *HHY, double *hmin);
+
+/* MASAS (subrutinas.f90:128-153): lumped mass M(npoin) (overwritten) */
+int cfd_masas(cfd_mesh *m, const double *area, double *M);
+
+/* deltat (subrutinas.f90:155-218): DT(nelem) clipped at 10*dtmin */
+int cfd_deltat(cfd_mesh *m, const cfd_consts *c, const double *area, const double *T,
+               const double *VEL_X, const double *VEL_Y, const double *W_X,
+               const double *W_Y, double *DT, double *dtmin);
+
+/* ESTAB (subrutinas.f90:332-446): SHOC, T_SUGN1..3 (nelem) */
+int cfd_estab(cfd_mesh *m, const cfd_consts *c, const double *U, const double *T,
+              const double *GAMM, const double *VEL_X, const double *VEL_Y,
+              const double *W_X, const double *W_Y, const double *dNx, const double *dNy,
+              double dtmin, double *shoc, double *t_sugn1, double *t_sugn2,
+              double *t_sugn3);
+
+/* calcRHS_mod::calcRHS (calcRHS.f90:4-154): ACCUMULATES into rhs(4,npoin).
+ * T(npoin) is the nodal temperature the reference reads from Mvariables. */
+int cfd_calcrhs(cfd_mesh *m, const cfd_consts *c, double *rhs, const double *U,
+                const double *theta, const double *T, const double *dNx, const double *dNy,
+                const double *area, const double *shoc, const double *dtl,
+                const double *t_sugn1, const double *t_sugn2, const double *t_sugn3);
+
+/* FUENTE (subrutinas.f90:1036-1094): ALE source, ACCUMULATES into rhs */
+int cfd_fuente(cfd_mesh *m, double *rhs, const double *U, const double *W_X,
+               const double *W_Y, const double *dNx, const double *dNy, const double *area,
+               const double *dtl);
+
+/* Boundary lists as left by dataLoader (dataLoader.f90:130-268). */
+typedef struct {
+    int nfixv;   const int *ifixv_node;   const double *rfixv_valuex, *rfixv_valuey;
+    int nfixrho; const int *ifixrho_node; const double *rfixrho_value;
+    int nfixt;   const int *ifixt_node;   const double *rfixt_value;
+    int nwall;   const int *wall;         /* wall(2,nwall) edges for NORMALES */
+} cfd_bc_lists;
+
+/* Mnormales::normales (subrutinas.f90:7-64): wall normals from X, Y; kept in
+ * the mesh context for cfd_rk (the reference keeps them module-private). */
+int cfd_normales(cfd_mesh *m, const cfd_bc_lists *bc, const double *X, const double *Y);
+
+/* RK (subrutinas.f90:645-849, NGAS=0): the reference passes these through
+ * modules; here every array is explicit (host, node-major). */
+typedef struct {
+    double *U1, *RHS, *T, *VEL_X, *VEL_Y, *RHO, *E, *P, *RMACH;
+    const double *U, *W_X, *W_Y, *M, *GAMM;
+} cfd_rk_nodes;
+typedef struct {
+    const double *dNx, *dNy, *area;
+    double *shoc, *t_sugn1, *t_sugn2, *t_sugn3;
+} cfd_rk_elems;
+int cfd_rk(cfd_mesh *m, const cfd_consts *c, const cfd_bc_lists *bc, int nrk, double dtmin,
+           const double *dtl, cfd_rk_nodes *nodes, cfd_rk_elems *elems);
+
+/* BiconjGrad::biCG (biconjGrad.f90:8-62): Jacobi-PCG on a CSR matrix with
+ * Dirichlet nodes; x is the initial guess in, the solution out.  Extensions
+ * the reference lacks: *iters (iterations run), *rz (final |r.z|). */
+int cfd_bicg(int npoin, const double *spMtx, const int *spIdx, const int *spRowptr,
+             const double *diagMtx, double *x, const double *b, const double *x_fix,
+             const int *x_fixIdx, int nfix, int *iters, double *rz);
+
+/* gcl_mod (gcl.f90:8-62): putW / putArea store the previous mesh velocity and
+ * areas in the mesh context; main updates M in place.  mode 0 reproduces the
+ * file as written (y-divergence built from W_x), mode 1 uses W_y. */
+int cfd_gcl_put_w(cfd_mesh *m, const double *W_x, const double *W_y);
+int cfd_gcl_put_area(cfd_mesh *m, const double *area);
+int cfd_gcl_main(cfd_mesh *m, double *M, const double *W_x, const double *W_y,
+                 const double *dNx, const double *dNy, const double *area, double dt,
+                 int mode);
+
+/* ---- device-resident session (ns2DComp.ALE.f90:138-281) ------------------ */
+typedef struct {
+    cfd_consts c;
+    int nrk;                 /* 4 (ns2DComp.ALE.f90:118)                     */
+    int moving;              /* MOVING flag: re-run geometry every step       */
+    int itlocal;             /* ITLOCAL (local time stepping blend)           */
+    double xref, yref;       /* XREF(1), YREF(1): pitch axis of force set 1   */
+    int nsetedges;           /* force set 1 edges                             */
+    const int *iset;         /* (3,nsetedges): n1 n2 elem                     */
+    int nmove, nfix_move;    /* moving / fixed-motion node counts             */
+    const int *ilaux;        /* moving nodes, then fixed-motion nodes         */
+    cfd_bc_lists bc;
+    const double *X, *Y;     /* initial coordinates (npoin)                   */
+    /* options beyond the reference (0 = reference behaviour) */
+    int motion;              /* 0: DISN(2) = pi/8 sin(10 t) (meshMove.f90:70)
+                                1: alpha(t) = motion_amp * sin(motion_omega t) */
+    double motion_amp, motion_omega;
+    int pcg_rel;             /* 0: |r.z| <= pcg_tol absolute (reference)
+                                1: |r.z| <= pcg_tol^2 * |r0.z0| of a zero guess */
+    double pcg_tol;          /* 0 means 1e-10                                 */
+    int pcg_maxit;           /* 0 means 1000                                  */
+    int gcl;                 /* -1: no GCL (reference); 0/1: gcl mode          */
+    int precision;           /* 0: FP64 element kernel; 1: FP32 element       */
+} cfd_case_desc;
+
+enum {
+    CFD_F_U = 0, CFD_F_U1, CFD_F_RHS, CFD_F_T, CFD_F_VEL_X, CFD_F_VEL_Y, CFD_F_W_X,
+    CFD_F_W_Y, CFD_F_X, CFD_F_Y, CFD_F_X1, CFD_F_Y1, CFD_F_RHO, CFD_F_E, CFD_F_P,
+    CFD_F_RMACH, CFD_F_M, CFD_F_GAMM, CFD_F_XPOS, CFD_F_YPOS, CFD_F_DTL, CFD_F_DT,
+    CFD_F_SHOC, CFD_F_T_SUGN1, CFD_F_T_SUGN2, CFD_F_T_SUGN3, CFD_F_AREA, CFD_F_DNX,
+    CFD_F_DNY, CFD_F_HH, CFD_F_HHX, CFD_F_HHY, CFD_F_LAP_SPARSE, CFD_F_LAP_DIAG,
+    CFD_NFIELDS
+};
+enum {
+    CFD_S_DTMIN = 0, CFD_S_TIME, CFD_S_DTMIN1, CFD_S_BANDERA, CFD_S_ITER, CFD_S_HMIN,
+    CFD_S_FX, CFD_S_FY, CFD_S_RM, CFD_S_DISN2, CFD_S_PCG_ITX, CFD_S_PCG_ITY,
+    CFD_S_PCG_RZX, CFD_S_PCG_RZY, CFD_NSCALARS
+};
+/* kernel timing slots (HIP events on the session stream) */
+enum {
+    CFD_K_ELEMENT = 0,  /* calcRHS+FUENTE element kernel                      */
+    CFD_K_STAGE,        /* gather + stage update                              */
+    CFD_K_ESTAB, CFD_K_DELTAT, CFD_K_PCG, CFD_K_GEOM, CFD_K_STEP, CFD_NKTIMERS
+};
+
+int cfd_session_create(cfd_session **out, cfd_mesh *m, const cfd_case_desc *d);
+int cfd_session_destroy(cfd_session *s);
+/* RESTART free-stream branch (ns2DComp.ALE.f90:404-420) and the start-up
+ * geometry (NORMALES, DERIV, MASAS, laplace: :88-100) */
+int cfd_session_init_freestream(cfd_session *s);
+/* nsteps time steps; returns after the work is queued (cfd_session_sync waits) */
+int cfd_session_step(cfd_session *s, int nsteps);
+int cfd_session_sync(cfd_session *s);
+/* field size: npoin, 4*npoin, nelem, 3*nelem or nnz doubles */
+int cfd_session_field_size(cfd_session *s, int field, long long *count);
+int cfd_session_get_field(cfd_session *s, int field, double *host);
+int cfd_session_set_field(cfd_session *s, int field, const double *host);
+int cfd_session_get_scalars(cfd_session *s, double *scalars /* CFD_NSCALARS */);
+int cfd_session_set_scalars(cfd_session *s, const double *scalars);
+/* per-slot kernel time: enable resets the accumulators */
+int cfd_session_timing(cfd_session *s, int enable);
+int cfd_session_kernel_time(cfd_session *s, int slot, double *total_ms, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFD_ABI_H */
