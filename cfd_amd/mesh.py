@@ -153,6 +153,7 @@ class Case:
     prandtl: float = 0.72
     cte_in: float = 1.0
     itlocal: int = 0
+    slip_body: bool = False      # body edges as slip walls (NORMALES) instead of no-slip
     # derived (filled by __post_init__) -------------------------------------
     fr: float = field(init=False)
     u_inf: float = field(init=False)
@@ -211,12 +212,19 @@ def _boundary_lists(cs: Case) -> dict:
     rfixrho = one * cs.rho_inf
     # inflow velocity on the outer ring, then no-slip on the body
     # (dataLoader.f90:149-176: inflow first, no-slip appended, NFIXV += NFIXVI)
-    ifixv = np.concatenate([outer, body]).astype(np.int32)
-    rvx = np.concatenate([one * cs.u_inf, np.zeros(m.nt)])
-    rvy = np.concatenate([one * cs.v_inf, np.zeros(m.nt)])
+    noslip = np.zeros(0, np.int32) if cs.slip_body else body
+    ifixv = np.concatenate([outer, noslip]).astype(np.int32)
+    rvx = np.concatenate([one * cs.u_inf, np.zeros(noslip.size)])
+    rvy = np.concatenate([one * cs.v_inf, np.zeros(noslip.size)])
     # fixed T: no-slip nodes first (TWALL), then the listed nodes (value*T_inf)
-    ifixt = np.concatenate([body, outer]).astype(np.int32)
-    rfixt = np.concatenate([np.full(m.nt, cs.twall), one * cs.t_inf])
+    ifixt = np.concatenate([noslip, outer]).astype(np.int32)
+    rfixt = np.concatenate([np.full(noslip.size, cs.twall), one * cs.t_inf])
+    # slip walls: body edges in the wall list (normal velocity removed by NORMALVEL)
+    if cs.slip_body:
+        it0 = np.arange(m.nt)
+        wall = np.stack([m.node(0, it0), m.node(0, it0 + 1)], axis=1).astype(np.int32)
+    else:
+        wall = np.zeros((0, 2), np.int32)
     # force set 1: body edges, listed so that FORCES gives the force on the body
     it = np.arange(m.nt)
     n1 = m.node(0, it + 1).astype(np.int32)
@@ -234,7 +242,7 @@ def _boundary_lists(cs: Case) -> dict:
         ifixrho_node=ifixrho, rfixrho_value=rfixrho,
         nfixvi=m.nt, ifixv_node=ifixv, rfixv_valuex=rvx, rfixv_valuey=rvy,
         ifixt_node=ifixt, rfixt_value=rfixt,
-        wall=np.zeros((0, 2), np.int32),
+        wall=wall,
         iset=iset, nset_numb=1,
         i_m=i_m, ifm=ifm, ilaux=ilaux,
         nmove=int(i_m.size), nfix_move=int(ifm.size),
