@@ -284,14 +284,19 @@ def test_bicg_direct_solve_and_single_free_node():
     # the reference's breakdown, which the oracle reproduces.
     free = int(np.flatnonzero(~fixed)[len(np.flatnonzero(~fixed)) // 2]) + 1
     others = np.setdiff1d(np.arange(1, n + 1), [free]).astype(np.int32)
-    vals = np.random.default_rng(2).standard_normal(others.size) * 1e-9
-    x = np.zeros(n)
-    o.bicg(x, vals, others)
-    xa = np.zeros(n)
-    xa[others - 1] = vals
+    vals = np.random.default_rng(2).standard_normal(others.size)
     k0, k1 = t["lap_rowptr"][free - 1], t["lap_rowptr"][free]
     cols, a = t["lap_idx"][k0:k1] - 1, s["LAP_SPARSE"][k0:k1]
     off = cols != free - 1
+    xa = np.zeros(n)
+    xa[others - 1] = vals
+    r0 = -(a[off] * xa[cols[off]]).sum()
+    d = a[~off][0]
+    assert d > 2.0
+    vals *= np.sqrt(1.5e-10) / abs(r0)        # r.r = 1.5e-10 >= tol, r.z = r.r/d < tol
+    x = np.zeros(n)
+    o.bicg(x, vals, others)
+    xa[others - 1] = vals
     expect = -(a[off] * xa[cols[off]]).sum() / a[~off][0]
     assert abs(x[free - 1] - expect) <= 1e-12 * abs(expect)
 
