@@ -174,7 +174,6 @@ struct cfd_mesh {
     Tables t;
     std::map<std::string, DBuf<double>> dscratch;
     DBuf<double> partials;
-    DBuf<unsigned> counter;
     DBuf<double> dscal;
     // NORMALES / RK state
     std::unique_ptr<BcDevice> bc;
@@ -243,8 +242,6 @@ int cfd_mesh_create(cfd_mesh** out, int nelem, int npoin, const int* inpoel) {
         CFD_HIP(hipStreamCreate(&m->s));
         build_tables(m->t, inpoel, nelem, npoin, m->s);
         m->partials.alloc(grid_for(std::max(nelem, npoin)));
-        m->counter.alloc(1);
-        m->counter.zero(m->s);
         m->dscal.alloc(8);
         m->sync();
         *out = m.release();
@@ -322,8 +319,7 @@ int cfd_deriv(cfd_mesh* m, const double* X, const double* Y, double* area, doubl
         double* h = m->scratch("HH", ne);
         double* hx = m->scratch("HHX", ne);
         double* hy = m->scratch("HHY", ne);
-        launch_deriv(m->t, dX, dY, a, nx, ny, h, hx, hy, m->dscal.p, m->partials.p, m->counter.p,
-                     m->s);
+        launch_deriv(m->t, dX, dY, a, nx, ny, h, hx, hy, m->dscal.p, m->partials.p, m->s);
         m->get(area, a, ne);
         m->get(dNx, nx, 3 * (size_t)ne);
         m->get(dNy, ny, 3 * (size_t)ne);
@@ -359,8 +355,7 @@ int cfd_deltat(cfd_mesh* m, const cfd_consts* c, const double* area, const doubl
         double* wx = m->put("WX", W_X, np);
         double* wy = m->put("WY", W_Y, np);
         double* dt = m->scratch("DT", ne);
-        launch_deltat(m->t, k, a, t, vx, vy, wx, wy, dt, m->dscal.p, m->partials.p, m->counter.p,
-                      m->s);
+        launch_deltat(m->t, k, a, t, vx, vy, wx, wy, dt, m->dscal.p, m->partials.p, m->s);
         m->get(DT, dt, ne);
         m->get(dtmin, (const double*)m->dscal.p, 1);
         m->sync();
@@ -629,8 +624,9 @@ struct cfd_session {
     DBuf<double> RHS, T, VX, VY, WX, WY, X, Y, X1, Y1, RHO, E, P, RMACH, M, GAMM, XPOS, YPOS;
     DBuf<double> DTL, DT, SHOC, T1, T2, T3, AREA, DNX, DNY, HH, HHX, HHY, AREA_OLD, WXO, WYO;
     DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, BZERO, dscal, partials;
+    // FP32 element path: binary32 shape gradients, ESTAB outputs, slots
+    DBuf<float> DNX32, DNY32, NU32, TA32, TB32, TC32, FAC32, c32, f32b;
     DBuf<int> ILAUX, ISET;
-    DBuf<unsigned> counter;
     BcDevice bc;
     PcgWork pcg;
     double sc[CFD_NSCALARS] = {0};
@@ -692,9 +688,17 @@ struct cfd_session {
         const Tables& t = m->t;
         bc.normales(X.p, Y.p, s);
         launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p, dscal.p + 1, partials.p,
-                     counter.p, s);
+                     s);
+        refresh_f32_geometry();
         launch_masas(t, AREA.p, M.p, s);
         launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
+    }
+
+    // binary32 copies of the shape gradients read by the FP32 element kernel
+    void refresh_f32_geometry() {
+        if (d.precision != 1) return;
+        launch_to_f32(DNX.p, DNX32.p, 3LL * ne, s);
+        launch_to_f32(DNY.p, DNY32.p, 3LL * ne, s);
     }
 
     double motion_angle(double time) const {
@@ -726,8 +730,7 @@ struct cfd_session {
         sc[CFD_S_ITER] += 1.0;
         // deltat and the DTMIN hysteresis (ns2DComp.ALE.f90:144-166)
         tb(CFD_K_DELTAT);
-        launch_deltat(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p, DT.p, dscal.p, partials.p,
-                      counter.p, s);
+        launch_deltat(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p, DT.p, dscal.p, partials.p, s);
         te(CFD_K_DELTAT);
         CFD_HIP(hipMemcpyAsync(hpin, dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
         CFD_HIP(hipStreamSynchronize(s));
@@ -754,24 +757,45 @@ struct cfd_session {
         const double time = sc[CFD_S_TIME];
 
         // RK (subrutinas.f90:645-849)
+        const bool f32 = d.precision == 1;
+        Elem32 o32;
+        if (f32) {
+            o32.nu = NU32.p;
+            o32.ta = TA32.p;
+            o32.tb = TB32.p;
+            o32.tc = TC32.p;
+            o32.fac = FAC32.p;
+            o32.area = AREA.p;
+            o32.dtl = DTL.p;
+            o32.cte = c.cte;
+        }
         tb(CFD_K_ESTAB);
         launch_estab(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin, SHOC.p,
-                     T1.p, T2.p, T3.p, s);
+                     T1.p, T2.p, T3.p, s, o32);
         te(CFD_K_ESTAB);
         const bool fu = fuente_needed();
         ElemIn in{U(), nullptr, T.p, DNX.p, DNY.p, AREA.p, SHOC.p, DTL.p, T1.p, T2.p, T3.p,
                   fu ? WX.p : nullptr, fu ? WY.p : nullptr};
+        ElemIn32 in32{U(), T.p, DNX32.p, DNY32.p, FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p,
+                      fu ? WX.p : nullptr, fu ? WY.p : nullptr};
         StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p, WX.p,
                       WY.p};
         const int nrk = d.nrk > 0 ? d.nrk : 4;
         for (int irk = 1; irk <= nrk; ++irk) {
             double rk_fact = 1.0 / (nrk + 1 - irk);
             tb(CFD_K_ELEMENT);
-            launch_element_rhs(t, c, in, c_calc.p, fu ? c_fuente.p : nullptr, s);
+            if (f32)
+                launch_element_rhs32(t, c, in32, c32.p, fu ? f32b.p : nullptr, s);
+            else
+                launch_element_rhs(t, c, in, c_calc.p, fu ? c_fuente.p : nullptr, s);
             te(CFD_K_ELEMENT);
             tb(CFD_K_STAGE);
-            launch_stage_update(t, c, bc.tab, rk_fact, c_calc.p, fu ? c_fuente.p : nullptr, sn,
-                                irk == nrk, s);
+            if (f32)
+                launch_stage_update32(t, c, bc.tab, rk_fact, c32.p, fu ? f32b.p : nullptr, sn,
+                                      irk == nrk, s);
+            else
+                launch_stage_update(t, c, bc.tab, rk_fact, c_calc.p, fu ? c_fuente.p : nullptr, sn,
+                                    irk == nrk, s);
             te(CFD_K_STAGE);
         }
 
@@ -798,7 +822,8 @@ struct cfd_session {
                                        hipMemcpyDeviceToDevice, s));
                 bc.normales(X.p, Y.p, s);
                 launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p, dscal.p + 1,
-                             partials.p, counter.p, s);
+                             partials.p, s);
+                refresh_f32_geometry();
                 launch_gcl(t, M.p, WX.p, WY.p, WXO.p, WYO.p, DNX.p, DNY.p, AREA.p, AREA_OLD.p, dtmin,
                            d.gcl, s);
                 launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
@@ -868,7 +893,7 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         CFD_ARG(d->nmove >= 0 && d->nfix_move >= 0 && d->nsetedges >= 0, "list sizes");
         CFD_ARG(d->nmove + d->nfix_move == 0 || d->ilaux, "ilaux");
         CFD_ARG(d->nsetedges == 0 || d->iset, "iset");
-        CFD_ARG(d->precision == 0, "only the FP64 element kernel is available in this build");
+        CFD_ARG(d->precision == 0 || d->precision == 1, "precision must be 0 (FP64) or 1 (FP32 element)");
         auto S = std::make_unique<cfd_session>();
         S->m = m;
         S->s = m->s;
@@ -902,8 +927,15 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->DNX.alloc(3 * (size_t)ne);
         S->DNY.alloc(3 * (size_t)ne);
         S->LAP.alloc(m->t.nnz);
-        S->c_calc.alloc(12 * (size_t)ne);
-        S->c_fuente.alloc(12 * (size_t)ne);
+        if (d->precision == 1) {
+            for (auto* b : {&S->DNX32, &S->DNY32}) b->alloc(3 * (size_t)ne);
+            for (auto* b : {&S->NU32, &S->TA32, &S->TB32, &S->TC32, &S->FAC32}) b->alloc(ne);
+            S->c32.alloc(12 * (size_t)ne);
+            S->f32b.alloc(12 * (size_t)ne);
+        } else {
+            S->c_calc.alloc(12 * (size_t)ne);
+            S->c_fuente.alloc(12 * (size_t)ne);
+        }
         S->POSAUX.alloc(std::max(1, nn));
         S->ILAUX.alloc(std::max(1, nn));
         S->ILAUX.upload(S->ilaux_h.data(), nn, s);
@@ -912,8 +944,6 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->dscal.alloc(8);
         S->dscal.zero(s);
         S->partials.alloc(grid_for(std::max(ne, np)));
-        S->counter.alloc(1);
-        S->counter.zero(s);
         // zero everything once (DXPOS/DYPOS keep zeros off the set nodes,
         // meshMove.f90:52-53; B stays zero, :91-95)
         for (auto* b : {&S->Ub[0], &S->Ub[1], &S->RHS, &S->T, &S->VX, &S->VY, &S->WX, &S->WY, &S->X1,
@@ -995,6 +1025,10 @@ int cfd_session_set_field(cfd_session* S, int f, const double* host) {
         b->upload(host, n, S->s);
         CFD_HIP(hipStreamSynchronize(S->s));
         if (f == CFD_F_W_X || f == CFD_F_W_Y || f == CFD_F_XPOS || f == CFD_F_YPOS) S->w_dirty = true;
+        if (f == CFD_F_DNX || f == CFD_F_DNY) {
+            S->refresh_f32_geometry();
+            CFD_HIP(hipStreamSynchronize(S->s));
+        }
     })
 }
 

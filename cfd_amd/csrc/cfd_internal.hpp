@@ -110,10 +110,11 @@ struct Consts {
     double gama, fr, fmu, fk, fcv, t_inf, rho_inf, u_inf, v_inf, cte, fsafe;
 };
 
-// deriv (subrutinas.f90:88-126); hmin_out: device double, written by the last block
+// deriv (subrutinas.f90:88-126); hmin_out: device double; partials holds
+// grid_for(nelem) doubles
 void launch_deriv(const Tables& t, const double* X, const double* Y, double* area, double* dNx,
                   double* dNy, double* HH, double* HHX, double* HHY, double* hmin_out,
-                  double* partials, unsigned* counter, hipStream_t s);
+                  double* partials, hipStream_t s);
 // MASAS (subrutinas.f90:128-153) as an ascending-element gather
 void launch_masas(const Tables& t, const double* area, double* M, hipStream_t s);
 // mLaplace.f90:7-58 values (structure comes from Tables)
@@ -127,13 +128,21 @@ void launch_gcl(const Tables& t, double* M, const double* Wx, const double* Wy,
 // deltat (subrutinas.f90:155-218): DT and dtmin (device), clipped at 10*dtmin
 void launch_deltat(const Tables& t, const Consts& c, const double* area, const double* T,
                    const double* VX, const double* VY, const double* WX, const double* WY,
-                   double* DT, double* dtmin_out, double* partials, unsigned* counter,
-                   hipStream_t s);
+                   double* DT, double* dtmin_out, double* partials, hipStream_t s);
+// Inputs of the FP32 element residual (nelem floats each), written by ESTAB
+// when nu != nullptr: nu = SHOC*CTE, tau1..3, fac = area*dtl/3.
+struct Elem32 {
+    float *nu = nullptr, *ta = nullptr, *tb = nullptr, *tc = nullptr, *fac = nullptr;
+    const double *area = nullptr, *dtl = nullptr;
+    double cte = 1.0;
+};
+
 // ESTAB (subrutinas.f90:332-446)
 void launch_estab(const Tables& t, const Consts& c, const double* U, const double* T,
                   const double* GAMM, const double* VX, const double* VY, const double* WX,
                   const double* WY, const double* dNx, const double* dNy, double dtmin,
-                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s);
+                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s,
+                  const Elem32& o32 = Elem32());
 
 // Element residual: calcRHS (calcRHS.f90:36-151) and, when WX != nullptr,
 // FUENTE (subrutinas.f90:1060-1091), written to per-slot buffers indexed
@@ -146,6 +155,16 @@ void launch_element_rhs(const Tables& t, const Consts& c, const ElemIn& in, doub
 
 // FUENTE alone into its slot buffer (uses U, dNx, dNy, area, dtl, WX, WY)
 void launch_fuente(const Tables& t, const ElemIn& in, double* c_fuente, hipStream_t s);
+
+// FP32 element residual (+ FUENTE when WX != nullptr), float4 slots
+struct ElemIn32 {
+    const double *U, *T;
+    const float *dNx, *dNy, *fac, *nu, *ta, *tb, *tc;
+    const double *WX, *WY;
+};
+void launch_element_rhs32(const Tables& t, const Consts& c, const ElemIn32& in, float* c_calc,
+                          float* c_fuente, hipStream_t s);
+void launch_to_f32(const double* a, float* b, long long n, hipStream_t s);
 
 // Deterministic node gather: rhs(i) = rhs_in(i) + sum calc contributions in
 // esup order, then + sum FUENTE contributions (reference accumulation order).
@@ -171,6 +190,10 @@ struct StageNodes {
 void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
                          const double* c_calc, const double* c_fuente, const StageNodes& nd,
                          bool write_rhs, hipStream_t s);
+// the same with FP32 slots (summed in FP64)
+void launch_stage_update32(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
+                           const float* c_calc, const float* c_fuente, const StageNodes& nd,
+                           bool write_rhs, hipStream_t s);
 
 // NORMALES (subrutinas.f90:7-64) on the BC table entries that are wall nodes
 void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, const int* wnode_ptr,
@@ -191,7 +214,6 @@ struct PcgWork {
     int n = 0, nblocks = 0;
     DBuf<double> r, z, y, pa, pb, partial;
     DBuf<double> scal;         // see pcg.hip for the slot meaning
-    DBuf<unsigned> counter;
     DBuf<int> fixflag;         // n: 1 on Dirichlet nodes
     void ensure(int npoin);
 };

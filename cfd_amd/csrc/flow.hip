@@ -30,8 +30,7 @@ __global__ void __launch_bounds__(kBlock)
 k_deltat(const int* __restrict__ inpoel, const double* __restrict__ area, const double* __restrict__ T,
          const double* __restrict__ VX, const double* __restrict__ VY, const double* __restrict__ WX,
          const double* __restrict__ WY, double* __restrict__ DT, int nelem, double t_inf,
-         double fsafe, double* __restrict__ dtmin_out, double* __restrict__ partials,
-         unsigned* __restrict__ counter) {
+         double fsafe, double* __restrict__ partials) {
     double dmin = 1.0e20;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += gridDim.x * blockDim.x) {
         const int* ip = inpoel + 3 * e;
@@ -60,13 +59,7 @@ k_deltat(const int* __restrict__ inpoel, const double* __restrict__ area, const 
         DT[e] = dtelem;
         if (dtelem < dmin) dmin = dtelem;
     }
-    if (block_partial(dmin, OpMin(), partials, counter)) {
-        double m = combine_partials(partials, gridDim.x, 1.0e20, OpMin());
-        if (threadIdx.x == 0) {
-            *dtmin_out = m;
-            *counter = 0;
-        }
-    }
+    block_partial(dmin, OpMin(), partials);
 }
 
 __global__ void k_deltat_clip(double* __restrict__ DT, const double* __restrict__ dtmin, int nelem) {
@@ -86,7 +79,7 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
         const double* __restrict__ W_Y, const double* __restrict__ dNx,
         const double* __restrict__ dNy, int nelem, double fr, double tinf, double rhoinf,
         double dtmin, double* __restrict__ shoc, double* __restrict__ t1, double* __restrict__ t2,
-        double* __restrict__ t3) {
+        double* __restrict__ t3, Elem32 o32) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nelem) return;
     int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
@@ -154,6 +147,14 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
     t1[e] = rrr;
     t2[e] = s2;
     t3[e] = s3;
+    if (o32.nu) {  // inputs of the FP32 element residual, rounded once here
+        double sh = (tr1 + tr1 * tr1) * 0.5 * (cc * cc) * zzz;
+        o32.nu[e] = (float)(sh * o32.cte);
+        o32.ta[e] = (float)rrr;
+        o32.tb[e] = (float)s2;
+        o32.tc[e] = (float)s3;
+        o32.fac[e] = (float)(o32.area[e] * o32.dtl[e] / 3.0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -381,36 +382,60 @@ k_element_rhs(ElemConsts kc, const int* __restrict__ inpoel, const int* __restri
     }
 }
 
+// Slot segments of a block of consecutive nodes are one contiguous range of
+// the slot buffer.  It is staged through LDS in chunks with coalesced loads
+// (consecutive lanes, consecutive slots); each node then adds its own slots
+// from LDS in ascending order.  (Strided per-node loads straight from HBM
+// re-fetched every line about twice: each wave's segment span was evicted
+// from L2 between its iterations.)
+constexpr int kChunk = 1024;
+
+template <class CT>  // double4 (FP64 contributions) or float4 (FP32, summed in FP64)
+__device__ inline void staged_sum(const CT* __restrict__ src, int s0, int s1, int b, int e, CT* sbuf,
+                                  double& r0, double& r1, double& r2, double& r3) {
+    for (int c0 = s0; c0 < s1; c0 += kChunk) {
+        const int c1 = min(c0 + kChunk, s1);
+        __syncthreads();
+        for (int q = c0 + (int)threadIdx.x; q < c1; q += (int)blockDim.x) sbuf[q - c0] = src[q];
+        __syncthreads();
+        const int lo = max(b, c0), hi = min(e, c1);
+        for (int s = lo; s < hi; ++s) {
+            CT c = sbuf[s - c0];
+            r0 = r0 + (double)c.x;
+            r1 = r1 + (double)c.y;
+            r2 = r2 + (double)c.z;
+            r3 = r3 + (double)c.w;
+        }
+    }
+}
+
 // rhs_out(i) = rhs_in(i) + calc slots in order (+ FUENTE slots in order)
+template <class CT>
 __global__ void __launch_bounds__(kBlock)
-k_gather_rhs(const int* __restrict__ esup2, const double* rhs_in,
-             const double* __restrict__ c_calc, const double* __restrict__ c_fuente,
-             double* rhs_out, int npoin) {  // rhs_in may alias rhs_out (same index only)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npoin) return;
+k_gather_rhs(const int* __restrict__ esup2, const double* rhs_in, const CT* __restrict__ c_calc,
+             const CT* __restrict__ c_fuente, double* rhs_out,
+             int npoin) {  // rhs_in may alias rhs_out (same index only)
+    __shared__ CT sbuf[kChunk];
+    const int i0 = blockIdx.x * kBlock, i1 = min(i0 + kBlock, npoin);
+    const int i = i0 + threadIdx.x;
+    const bool act = i < npoin;
+    const int s0 = esup2[i0], s1 = esup2[i1];
+    const int b = act ? esup2[i] : 0, e = act ? esup2[i + 1] : 0;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    if (rhs_in) {
+    if (act && rhs_in) {
         double4 v = ld4(rhs_in + 4 * (long long)i);
         r0 = v.x; r1 = v.y; r2 = v.z; r3 = v.w;
     }
-    const int b = esup2[i], e = esup2[i + 1];
-    for (int s = b; s < e; ++s) {
-        double4 c = ld4(c_calc + 4 * (long long)s);
-        r0 = r0 + c.x; r1 = r1 + c.y; r2 = r2 + c.z; r3 = r3 + c.w;
-    }
-    if (c_fuente)
-        for (int s = b; s < e; ++s) {
-            double4 c = ld4(c_fuente + 4 * (long long)s);
-            r0 = r0 + c.x; r1 = r1 + c.y; r2 = r2 + c.z; r3 = r3 + c.w;
-        }
-    st4(rhs_out + 4 * (long long)i, r0, r1, r2, r3);
+    staged_sum(c_calc, s0, s1, b, e, sbuf, r0, r1, r2, r3);
+    if (c_fuente) staged_sum(c_fuente, s0, s1, b, e, sbuf, r0, r1, r2, r3);
+    if (act) st4(rhs_out + 4 * (long long)i, r0, r1, r2, r3);
 }
 
 // Gather + one RK stage for one node (subrutinas.f90:679-826, NGAS = 0)
-template <bool kFuente>
+template <class CT, bool kFuente>
 __global__ void __launch_bounds__(kBlock)
-k_stage(const int* __restrict__ esup2, const double* __restrict__ c_calc,
-        const double* __restrict__ c_fuente, double rk_fact, double fr,
+k_stage(const int* __restrict__ esup2, const CT* __restrict__ c_calc,
+        const CT* __restrict__ c_fuente, double rk_fact, double fr,
         const double* __restrict__ U, const double* __restrict__ M, const double* __restrict__ GAMM,
         const double* __restrict__ WX, const double* __restrict__ WY,
         const int* __restrict__ bcidx, const int* __restrict__ bcflags,
@@ -418,19 +443,16 @@ k_stage(const int* __restrict__ esup2, const double* __restrict__ c_calc,
         double* __restrict__ T, double* __restrict__ VX, double* __restrict__ VY,
         double* __restrict__ RHO, double* __restrict__ E, double* __restrict__ P,
         double* __restrict__ RMACH, int npoin) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npoin) return;
+    __shared__ CT sbuf[kChunk];
+    const int i0 = blockIdx.x * kBlock, i1 = min(i0 + kBlock, npoin);
+    const int i = i0 + threadIdx.x;
+    const bool act = i < npoin;
+    const int s0 = esup2[i0], s1 = esup2[i1];
+    const int b = act ? esup2[i] : 0, e = act ? esup2[i + 1] : 0;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    const int b = esup2[i], e = esup2[i + 1];
-    for (int s = b; s < e; ++s) {
-        double4 c = ld4(c_calc + 4 * (long long)s);
-        r0 = r0 + c.x; r1 = r1 + c.y; r2 = r2 + c.z; r3 = r3 + c.w;
-    }
-    if (kFuente)
-        for (int s = b; s < e; ++s) {
-            double4 c = ld4(c_fuente + 4 * (long long)s);
-            r0 = r0 + c.x; r1 = r1 + c.y; r2 = r2 + c.z; r3 = r3 + c.w;
-        }
+    staged_sum(c_calc, s0, s1, b, e, sbuf, r0, r1, r2, r3);
+    if (kFuente) staged_sum(c_fuente, s0, s1, b, e, sbuf, r0, r1, r2, r3);
+    if (!act) return;
     if (RHS) st4(RHS + 4 * (long long)i, r0, r1, r2, r3);
     double4 u = ld4(U + 4 * (long long)i);
     double f = rk_fact / M[i];
@@ -471,26 +493,194 @@ k_stage(const int* __restrict__ esup2, const double* __restrict__ c_calc,
     st4(U1 + 4 * (long long)i, rho, vx * rho, vy * rho, en * rho);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 element residual (same equations as calcRHS.f90:37-141 with theta = 0,
+// i.e. as RK calls it).  Arithmetic in binary32 with contraction; the sums
+// over the three quadrature points are taken once per element:
+//   Galerkin  sum_k N(j,k) A_k         = (sum_k A_k - A_j) / 2
+//   SUPG      tau_j (Nx_j sum_k A1 A_k + Ny_j sum_k A2 A_k)
+//   shock     3 nu (Nx_j Ux + Ny_j Uy)
+//   viscous   Nx_j sum_k K1_k + Ny_j sum_k K2_k
+// Gradients and quadrature-point states come from FP64 node values: the
+// gradient uses differences to node 1, (U_j - U_1) N_j, so a uniform state
+// gives exactly zero despite the binary32 shape gradients.  Contributions are
+// float4 slots summed in FP64 by the stage gather.
+// ---------------------------------------------------------------------------
+struct ElemConsts32 {
+    float g, Cv, lambda_ref, mu_ref, T_inf;
+    int viscous;
+};
+
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <bool kFuente>
+__global__ void __launch_bounds__(kBlock)
+k_element_rhs32(ElemConsts32 kc, const int* __restrict__ inpoel, const int* __restrict__ epos,
+                const double* __restrict__ U, const double* __restrict__ T,
+                const float* __restrict__ dNx, const float* __restrict__ dNy,
+                const float* __restrict__ fac, const float* __restrict__ nu_,
+                const float* __restrict__ ta, const float* __restrict__ tb,
+                const float* __restrict__ tc, const double* __restrict__ WX,
+                const double* __restrict__ WY, float4* __restrict__ c_calc,
+                float4* __restrict__ c_fuente, int nelem) {
+#pragma clang fp contract(fast)
+    __shared__ int s_in[3 * kBlock];
+    __shared__ int s_pos[3 * kBlock];
+    __shared__ float s_nx[3 * kBlock];
+    __shared__ float s_ny[3 * kBlock];
+    const int e0 = blockIdx.x * kBlock;
+    const int nloc = min(kBlock, nelem - e0);
+    for (int q = threadIdx.x; q < 3 * nloc; q += kBlock) {
+        s_in[q] = inpoel[3 * (long long)e0 + q];
+        s_pos[q] = epos[3 * (long long)e0 + q];
+        s_nx[q] = dNx[3 * (long long)e0 + q];
+        s_ny[q] = dNy[3 * (long long)e0 + q];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x >= nloc) return;
+    const int e = e0 + threadIdx.x;
+    const int l = 3 * threadIdx.x;
+    const int p0 = s_in[l] - 1, p1 = s_in[l + 1] - 1, p2 = s_in[l + 2] - 1;
+    const float Nx[3] = {s_nx[l], s_nx[l + 1], s_nx[l + 2]};
+    const float Ny[3] = {s_ny[l], s_ny[l + 1], s_ny[l + 2]};
+    const double4 u0 = ld4(U + 4 * (long long)p0), u1 = ld4(U + 4 * (long long)p1),
+                  u2 = ld4(U + 4 * (long long)p2);
+    const double a0[4] = {u0.x, u0.y, u0.z, u0.w};
+    const double a1[4] = {u1.x, u1.y, u1.z, u1.w};
+    const double a2[4] = {u2.x, u2.y, u2.z, u2.w};
+    float Ux[4], Uy[4], Q[3][4];  // gradients; states at the midpoint opposite node k
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float d1 = (float)(a1[q] - a0[q]), d2 = (float)(a2[q] - a0[q]);
+        Ux[q] = d1 * Nx[1] + d2 * Nx[2];
+        Uy[q] = d1 * Ny[1] + d2 * Ny[2];
+        Q[0][q] = (float)(0.5 * (a1[q] + a2[q]));
+        Q[1][q] = (float)(0.5 * (a0[q] + a2[q]));
+        Q[2][q] = (float)(0.5 * (a0[q] + a1[q]));
+    }
+    const float ar = fac[e];
+    const float tau[3] = {ta[e], tb[e], tc[e]};
+    const float nu = nu_[e];
+    const float g = kc.g, gm1 = g - 1.0f, Cv = kc.Cv;
+    float mu = 0.0f, lambda = 0.0f;
+    if (kc.viscous) {
+        const float Tavg = (float)((T[p0] + T[p1] + T[p2]) * (1.0 / 3.0));
+        float x = Tavg / kc.T_inf;
+        float pw = x * __builtin_sqrtf(x);
+        mu = kc.mu_ref * pw * (kc.T_inf + 110.0f) * rcp32(Tavg + 110.0f);
+        lambda = kc.lambda_ref * pw * (kc.T_inf + 194.0f) * rcp32(Tavg + 194.0f);
+    }
+    float A[3][4], S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0}, V1[4] = {0, 0, 0, 0},
+                   V2[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float rho = Q[k][0], ir = rcp32(rho);
+        const float v1 = Q[k][1] * ir, v2 = Q[k][2] * ir, en = Q[k][3] * ir;
+        const float vsq = v1 * v1 + v2 * v2, h = vsq * gm1 - en * g;
+        float* a = A[k];
+        a[0] = Ux[1] + Uy[2];
+        a[1] = 0.5f * Ux[0] * (vsq * gm1 - 2.0f * v1 * v1) - Ux[1] * v1 * (g - 3.0f) -
+               Ux[2] * v2 * gm1 + Ux[3] * gm1 - Uy[0] * v1 * v2 + Uy[1] * v2 + Uy[2] * v1;
+        a[2] = -Ux[0] * v1 * v2 + Ux[1] * v2 + Ux[2] * v1 +
+               0.5f * Uy[0] * (vsq * gm1 - 2.0f * v2 * v2) - Uy[1] * v1 * gm1 -
+               Uy[2] * v2 * (g - 3.0f) + Uy[3] * gm1;
+        a[3] = Ux[0] * v1 * h - 0.5f * Ux[1] * (vsq * gm1 - 2.0f * en * g + 2.0f * v1 * v1 * gm1) -
+               Ux[2] * v1 * v2 * gm1 + Ux[3] * g * v1 + Uy[0] * v2 * h - Uy[1] * v1 * v2 * gm1 -
+               0.5f * Uy[2] * (vsq * gm1 - 2.0f * en * g + 2.0f * v2 * v2 * gm1) + Uy[3] * g * v2;
+        const float c12 = -v1 * v2 * a[0] + v1 * a[2] + v2 * a[1];
+        S1[0] += a[1];
+        S1[1] += v1 * (3.0f - g) * a[1] - v2 * gm1 * a[2] + gm1 * a[3] +
+                 (0.5f * vsq * gm1 - v1 * v1) * a[0];
+        S1[2] += c12;
+        S1[3] += g * v1 * a[3] - v1 * v2 * gm1 * a[2] + v1 * h * a[0] +
+                 (-0.5f * vsq * gm1 + en * g - v1 * v1 * gm1) * a[1];
+        S2[0] += a[2];
+        S2[1] += c12;
+        S2[2] += -v1 * gm1 * a[1] + v2 * (3.0f - g) * a[2] + gm1 * a[3] +
+                 (0.5f * vsq * gm1 - v2 * v2) * a[0];
+        S2[3] += g * v2 * a[3] - v1 * v2 * gm1 * a[1] + v2 * h * a[0] +
+                 (-0.5f * vsq * gm1 + en * g - v2 * v2 * gm1) * a[2];
+        if (kc.viscous) {
+            const float cm = Cv * mu, ir3cv = ir * (1.0f / 3.0f) * rcp32(Cv);
+            const float sh = mu * (-Ux[0] * v2 + Ux[2] - Uy[0] * v1 + Uy[1]) * ir;
+            V1[1] += (2.0f / 3.0f) * mu * (-2.0f * Ux[0] * v1 + 2.0f * Ux[1] + Uy[0] * v2 - Uy[2]) * ir;
+            V1[2] += sh;
+            V1[3] += (cm * (-Uy[0] * v1 * v2 + 3.0f * Uy[1] * v2 - 2.0f * Uy[2] * v1) -
+                      Ux[0] * (cm * (3.0f * vsq + v1 * v1) - 3.0f * lambda * (vsq - en)) +
+                      Ux[1] * v1 * (4.0f * cm - 3.0f * lambda) + 3.0f * Ux[2] * v2 * (cm - lambda) +
+                      3.0f * Ux[3] * lambda) * ir3cv;
+            V2[1] += sh;
+            V2[2] += (2.0f / 3.0f) * mu * (Ux[0] * v1 - Ux[1] - 2.0f * Uy[0] * v2 + 2.0f * Uy[2]) * ir;
+            V2[3] += (cm * (-Ux[0] * v1 * v2 - 2.0f * Ux[1] * v2 + 3.0f * Ux[2] * v1) -
+                      Uy[0] * (cm * (3.0f * vsq + v2 * v2) - 3.0f * lambda * (vsq - en)) +
+                      3.0f * Uy[1] * v1 * (cm - lambda) + Uy[2] * v2 * (4.0f * cm - 3.0f * lambda) +
+                      3.0f * Uy[3] * lambda) * ir3cv;
+        }
+    }
+    const int pos[3] = {s_pos[l], s_pos[l + 1], s_pos[l + 2]};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        float r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float gal = 0.5f * ((A[0][q] + A[1][q] + A[2][q]) - A[j][q]);
+            float supg = tau[j] * (Nx[j] * S1[q] + Ny[j] * S2[q]);
+            float shk = 3.0f * nu * (Nx[j] * Ux[q] + Ny[j] * Uy[q]);
+            float vis = Nx[j] * V1[q] + Ny[j] * V2[q];
+            r[q] = ar * (gal + supg + shk + vis);
+        }
+        c_calc[pos[j]] = make_float4(r[0], r[1], r[2], r[3]);
+    }
+    if (kFuente) {
+        // subrutinas.f90:1064-1078: -AR sum_k sp(j,k) (Ux wx_k + Uy wy_k)
+        const float w_x[3] = {(float)WX[p0], (float)WX[p1], (float)WX[p2]};
+        const float w_y[3] = {(float)WY[p0], (float)WY[p1], (float)WY[p2]};
+        // sp(:,1) = (1/2,1/2,0), sp(:,2) = (0,1/2,1/2), sp(:,3) = (1/2,0,1/2)
+        const float wx[3] = {0.5f * (w_x[0] + w_x[1]), 0.5f * (w_x[1] + w_x[2]), 0.5f * (w_x[0] + w_x[2])};
+        const float wy[3] = {0.5f * (w_y[0] + w_y[1]), 0.5f * (w_y[1] + w_y[2]), 0.5f * (w_y[0] + w_y[2])};
+        const float sp[3][3] = {{0.5f, 0.0f, 0.5f}, {0.5f, 0.5f, 0.0f}, {0.0f, 0.5f, 0.5f}};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float r[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc += sp[j][k] * (Ux[q] * wx[k] + Uy[q] * wy[k]);
+                r[q] = -ar * acc;
+            }
+            c_fuente[pos[j]] = make_float4(r[0], r[1], r[2], r[3]);
+        }
+    }
+}
+
+__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 void launch_deltat(const Tables& t, const Consts& c, const double* area, const double* T,
                    const double* VX, const double* VY, const double* WX, const double* WY,
-                   double* DT, double* dtmin_out, double* partials, unsigned* counter,
-                   hipStream_t s) {
-    k_deltat<<<red_grid(t.nelem), kBlock, 0, s>>>(t.inpoel.p, area, T, VX, VY, WX, WY, DT, t.nelem,
-                                                  c.t_inf, c.fsafe, dtmin_out, partials, counter);
-    k_deltat_clip<<<grid_for(t.nelem), kBlock, 0, s>>>(DT, dtmin_out, t.nelem);
+                   double* DT, double* dtmin_out, double* partials, hipStream_t s) {
+    const unsigned g = grid_for(t.nelem);
+    k_deltat<<<g, kBlock, 0, s>>>(t.inpoel.p, area, T, VX, VY, WX, WY, DT, t.nelem, c.t_inf, c.fsafe,
+                                  partials);
+    k_min_finalize<<<1, 256, 0, s>>>(partials, (int)g, 1.0e20, __builtin_huge_val(), dtmin_out);
+    k_deltat_clip<<<g, kBlock, 0, s>>>(DT, dtmin_out, t.nelem);
     CFD_HIP(hipGetLastError());
 }
 
 void launch_estab(const Tables& t, const Consts& c, const double* U, const double* T,
                   const double* GAMM, const double* VX, const double* VY, const double* WX,
                   const double* WY, const double* dNx, const double* dNy, double dtmin,
-                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s) {
+                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s,
+                  const Elem32& o32) {
     k_estab<<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy,
                                                  t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin, shoc, t1,
-                                                 t2, t3);
+                                                 t2, t3, o32);
     CFD_HIP(hipGetLastError());
 }
 
@@ -533,25 +723,69 @@ void launch_fuente(const Tables& t, const ElemIn& in, double* c_fuente, hipStrea
 
 void launch_gather_rhs(const Tables& t, const double* rhs_in, const double* c_calc,
                        const double* c_fuente, double* rhs_out, hipStream_t s) {
-    k_gather_rhs<<<grid_for(t.npoin), kBlock, 0, s>>>(t.esup2.p, rhs_in, c_calc, c_fuente, rhs_out,
-                                                      t.npoin);
+    k_gather_rhs<double4><<<grid_for(t.npoin), kBlock, 0, s>>>(
+        t.esup2.p, rhs_in, reinterpret_cast<const double4*>(c_calc),
+        reinterpret_cast<const double4*>(c_fuente), rhs_out, t.npoin);
+    CFD_HIP(hipGetLastError());
+}
+
+template <class CT>
+static void stage_launch(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
+                         const CT* cc, const CT* cf, const StageNodes& nd, bool write_rhs,
+                         hipStream_t s) {
+    dim3 g(grid_for(t.npoin)), b(kBlock);
+    double* rhs = write_rhs ? nd.RHS : nullptr;
+    if (cf)
+        k_stage<CT, true><<<g, b, 0, s>>>(t.esup2.p, cc, cf, rk_fact, c.fr, nd.U, nd.M, nd.GAMM, nd.WX,
+                                          nd.WY, bc.bcidx.p, bc.flags.p, bc.vals.p, nd.U1, rhs, nd.T,
+                                          nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH, t.npoin);
+    else
+        k_stage<CT, false><<<g, b, 0, s>>>(t.esup2.p, cc, cf, rk_fact, c.fr, nd.U, nd.M, nd.GAMM,
+                                           nd.WX, nd.WY, bc.bcidx.p, bc.flags.p, bc.vals.p, nd.U1, rhs,
+                                           nd.T, nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH, t.npoin);
     CFD_HIP(hipGetLastError());
 }
 
 void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
                          const double* c_calc, const double* c_fuente, const StageNodes& nd,
                          bool write_rhs, hipStream_t s) {
-    dim3 g(grid_for(t.npoin)), b(kBlock);
-    double* rhs = write_rhs ? nd.RHS : nullptr;
-    if (c_fuente)
-        k_stage<true><<<g, b, 0, s>>>(t.esup2.p, c_calc, c_fuente, rk_fact, c.fr, nd.U, nd.M, nd.GAMM,
-                                      nd.WX, nd.WY, bc.bcidx.p, bc.flags.p, bc.vals.p, nd.U1, rhs,
-                                      nd.T, nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH, t.npoin);
+    stage_launch(t, c, bc, rk_fact, reinterpret_cast<const double4*>(c_calc),
+                 reinterpret_cast<const double4*>(c_fuente), nd, write_rhs, s);
+}
+
+void launch_stage_update32(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
+                           const float* c_calc, const float* c_fuente, const StageNodes& nd,
+                           bool write_rhs, hipStream_t s) {
+    stage_launch(t, c, bc, rk_fact, reinterpret_cast<const float4*>(c_calc),
+                 reinterpret_cast<const float4*>(c_fuente), nd, write_rhs, s);
+}
+
+void launch_element_rhs32(const Tables& t, const Consts& c, const ElemIn32& in, float* c_calc,
+                          float* c_fuente, hipStream_t s) {
+    ElemConsts32 k;
+    k.g = (float)c.gama;
+    k.Cv = (float)c.fcv;
+    k.lambda_ref = (float)c.fk;
+    k.mu_ref = (float)c.fmu;
+    k.T_inf = (float)c.t_inf;
+    k.viscous = c.fmu > 2.2250738585072014e-308 ? 1 : 0;
+    dim3 g(grid_for(t.nelem)), b(kBlock);
+    float4* cc = reinterpret_cast<float4*>(c_calc);
+    float4* cf = reinterpret_cast<float4*>(c_fuente);
+    if (in.WX)
+        k_element_rhs32<true><<<g, b, 0, s>>>(k, t.inpoel.p, t.epos.p, in.U, in.T, in.dNx, in.dNy,
+                                              in.fac, in.nu, in.ta, in.tb, in.tc, in.WX, in.WY, cc, cf,
+                                              t.nelem);
     else
-        k_stage<false><<<g, b, 0, s>>>(t.esup2.p, c_calc, c_fuente, rk_fact, c.fr, nd.U, nd.M,
-                                       nd.GAMM, nd.WX, nd.WY, bc.bcidx.p, bc.flags.p, bc.vals.p,
-                                       nd.U1, rhs, nd.T, nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH,
-                                       t.npoin);
+        k_element_rhs32<false><<<g, b, 0, s>>>(k, t.inpoel.p, t.epos.p, in.U, in.T, in.dNx, in.dNy,
+                                               in.fac, in.nu, in.ta, in.tb, in.tc, nullptr, nullptr,
+                                               cc, nullptr, t.nelem);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_to_f32(const double* a, float* b, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    k_to_f32<<<grid_for((size_t)n), kBlock, 0, s>>>(a, b, n);
     CFD_HIP(hipGetLastError());
 }
 

@@ -18,7 +18,7 @@ __global__ void __launch_bounds__(kBlock)
 k_deriv(const int* __restrict__ inpoel, const double* __restrict__ X, const double* __restrict__ Y,
         double* __restrict__ area, double* __restrict__ dNx, double* __restrict__ dNy,
         double* __restrict__ HH, double* __restrict__ HHX, double* __restrict__ HHY, int nelem,
-        double* __restrict__ hmin_out, double* __restrict__ partials, unsigned* __restrict__ counter) {
+        double* __restrict__ partials) {
     double hm = __builtin_huge_val();
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += gridDim.x * blockDim.x) {
         int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
@@ -44,14 +44,7 @@ k_deriv(const int* __restrict__ inpoel, const double* __restrict__ X, const doub
         HHX[e] = fabs(mx);
         HHY[e] = fabs(my);
     }
-    if (block_partial(hm, OpMin(), partials, counter)) {
-        double m = combine_partials(partials, gridDim.x, __builtin_huge_val(), OpMin());
-        if (threadIdx.x == 0) {
-            if (m > 1.0e10) m = 1.0e10;
-            *hmin_out = m;
-            *counter = 0;
-        }
-    }
+    block_partial(hm, OpMin(), partials);
 }
 
 // subrutinas.f90:137-152: M(i) = sum of area/3 over i's elements, ascending
@@ -219,9 +212,11 @@ __global__ void k_transf(int nedge, const int* __restrict__ iset, const double* 
 
 void launch_deriv(const Tables& t, const double* X, const double* Y, double* area, double* dNx,
                   double* dNy, double* HH, double* HHX, double* HHY, double* hmin_out,
-                  double* partials, unsigned* counter, hipStream_t s) {
-    k_deriv<<<red_grid(t.nelem), kBlock, 0, s>>>(t.inpoel.p, X, Y, area, dNx, dNy, HH, HHX, HHY,
-                                                 t.nelem, hmin_out, partials, counter);
+                  double* partials, hipStream_t s) {
+    const unsigned g = grid_for(t.nelem);
+    k_deriv<<<g, kBlock, 0, s>>>(t.inpoel.p, X, Y, area, dNx, dNy, HH, HHX, HHY, t.nelem, partials);
+    // hmin = minval(HH), capped at 1e10 (subrutinas.f90:124-125)
+    k_min_finalize<<<1, 256, 0, s>>>(partials, (int)g, __builtin_huge_val(), 1.0e10, hmin_out);
     CFD_HIP(hipGetLastError());
 }
 

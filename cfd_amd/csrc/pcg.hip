@@ -8,16 +8,17 @@
 //     k++; r -= alfa y; z = r/diag; e = r.z; beta = e/e_old; p = beta p + z
 //     y = A p (fix rows 1e30 p); alfa = e/p.y; x += alfa p; e_old = e
 //
-// On the device one iteration is two kernels:
+// On the device one iteration is two full-grid kernels, each followed by a
+// one-block finalize that combines the block partials and updates the
+// scalars in device memory:
 //   A: x += alfa p (the previous iteration's update), r -= alfa y, z = r/diag,
-//      block partials of r.z; the last block forms e and beta.
+//      partials of r.z;                       finalize: e, beta, k.
 //   B: each row forms p_new = beta p_old + z for its columns on the fly, y = A
-//      p_new, stores its own p_new (double-buffered p), block partials of p.y;
-//      the last block forms alfa and the loop test.
-// Scalars live in device memory, so the host only polls a done flag every
-// few iterations.  The kernels run a fixed grid-stride grid, and dot products
-// combine block partials in a fixed tree, so the iteration sequence is
-// deterministic (not the serial summation order of the reference).
+//      p_new, stores its own p_new (double-buffered p), partials of p.y;
+//                                             finalize: alfa, the loop test.
+// The host polls a done flag every few iterations.  Dot products combine
+// block partials in a fixed tree, so the iteration sequence is deterministic
+// (not the serial summation order of the reference).
 #include "cfd_internal.hpp"
 #include "reduce.hpp"
 
@@ -27,11 +28,9 @@ namespace {
 // scal[] slots
 enum { S_ALFA = 0, S_BETA, S_EOLD, S_ENEW, S_PY, S_RR, S_K, S_DONE, S_STOP, S_CONV, S_TOL, S_MAXIT };
 
-#define GRID_STRIDE(i, n) \
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
-
 __global__ void k_fix_init(int n, int* __restrict__ fixflag) {
-    GRID_STRIDE(i, n) fixflag[i] = 0;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) fixflag[i] = 0;
 }
 
 __global__ void k_fix_set(int nfix, const int* __restrict__ idx, const double* __restrict__ x_fix,
@@ -49,14 +48,15 @@ __global__ void k_scal_init(double* __restrict__ scal, double tol, int maxit) {
     scal[S_MAXIT] = (double)maxit;
 }
 
-// r = -1*(A x) + b with fixed rows y = 1e30 x and r(fix) = 0; partial r.r
+// r = -1*(A x) + b with fixed rows y = 1e30 x and r(fix) = 0; partials of r.r
 __global__ void __launch_bounds__(kBlock)
 k_residual(int n, const double* __restrict__ A, const int* __restrict__ idx,
            const int* __restrict__ rowptr, const int* __restrict__ fixflag,
            const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
-           double* __restrict__ scal, double* __restrict__ partials, unsigned* __restrict__ counter) {
+           double* __restrict__ partials) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
-    GRID_STRIDE(i, n) {
+    if (i < n) {
         double ri;
         if (fixflag[i]) {
             ri = 0.0;
@@ -66,10 +66,14 @@ k_residual(int n, const double* __restrict__ A, const int* __restrict__ idx,
             ri = -1.0 * dot + b[i];
         }
         r[i] = ri;
-        v = v + ri * ri;
+        v = ri * ri;
     }
-    if (!block_partial(v, OpSum(), partials, counter)) return;
-    double rr = combine_partials(partials, gridDim.x, 0.0, OpSum());
+    block_partial(v, OpSum(), partials);
+}
+
+__global__ void __launch_bounds__(256)
+k_fin_residual(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
+    double rr = combine_partials(partials, nb, 0.0, OpSum());
     if (threadIdx.x == 0) {
         scal[S_RR] = rr;
         if (rr < scal[S_TOL]) {
@@ -77,62 +81,71 @@ k_residual(int n, const double* __restrict__ A, const int* __restrict__ idx,
             scal[S_CONV] = 1.0;
             scal[S_EOLD] = rr;
         }
-        *counter = 0;
     }
 }
 
-// first preconditioned residual: z = r/diag, e = r.z
+// first preconditioned residual: z = r/diag, partials of r.z
 __global__ void __launch_bounds__(kBlock)
 k_first_z(int n, const double* __restrict__ r, const double* __restrict__ diag,
-          double* __restrict__ z, double* __restrict__ scal, double* __restrict__ partials,
-          unsigned* __restrict__ counter) {
+          double* __restrict__ z, const double* __restrict__ scal, double* __restrict__ partials) {
     if (scal[S_DONE] != 0.0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
-    GRID_STRIDE(i, n) {
+    if (i < n) {
         double zi = r[i] / diag[i];
         z[i] = zi;
-        v = v + r[i] * zi;
+        v = r[i] * zi;
     }
-    if (!block_partial(v, OpSum(), partials, counter)) return;
-    double e = combine_partials(partials, gridDim.x, 0.0, OpSum());
-    if (threadIdx.x == 0) {
-        scal[S_ENEW] = e;
-        *counter = 0;
-    }
+    block_partial(v, OpSum(), partials);
+}
+
+__global__ void __launch_bounds__(256)
+k_fin_first_z(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
+    if (scal[S_DONE] != 0.0) return;
+    double e = combine_partials(partials, nb, 0.0, OpSum());
+    if (threadIdx.x == 0) scal[S_ENEW] = e;
 }
 
 // Kernel A of an iteration (see the file header)
 __global__ void __launch_bounds__(kBlock)
 k_iter_a(int n, double* __restrict__ x, const double* __restrict__ pcur, double* __restrict__ r,
          const double* __restrict__ y, const double* __restrict__ diag, double* __restrict__ z,
-         double* __restrict__ scal, double* __restrict__ partials, unsigned* __restrict__ counter) {
+         const double* __restrict__ scal, double* __restrict__ partials) {
     if (scal[S_DONE] != 0.0) return;
     const double alfa = scal[S_ALFA];
     const bool stop = scal[S_STOP] != 0.0;
-    double v = 0.0;
-    GRID_STRIDE(i, n) {
-        x[i] = alfa * pcur[i] + x[i];
-        if (!stop) {
-            double ri = -alfa * y[i] + r[i];
-            r[i] = ri;
-            double zi = ri / diag[i];
-            z[i] = zi;
-            v = v + ri * zi;
-        }
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (stop) {  // the loop test failed: only the last x update remains
+        if (i < n) x[i] = alfa * pcur[i] + x[i];
+        return;
     }
-    if (!block_partial(v, OpSum(), partials, counter)) return;
-    double e = combine_partials(partials, gridDim.x, 0.0, OpSum());
-    if (threadIdx.x == 0) {
-        if (stop) {
+    double v = 0.0;
+    if (i < n) {
+        x[i] = alfa * pcur[i] + x[i];
+        double ri = -alfa * y[i] + r[i];
+        r[i] = ri;
+        double zi = ri / diag[i];
+        z[i] = zi;
+        v = ri * zi;
+    }
+    block_partial(v, OpSum(), partials);
+}
+
+__global__ void __launch_bounds__(256)
+k_fin_a(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
+    if (scal[S_DONE] != 0.0) return;
+    if (scal[S_STOP] != 0.0) {
+        if (threadIdx.x == 0) {
             scal[S_DONE] = 1.0;
-            double eo = scal[S_EOLD];
-            scal[S_CONV] = fabs(eo) > scal[S_TOL] ? 0.0 : 1.0;
-        } else {
-            scal[S_K] = scal[S_K] + 1.0;
-            scal[S_ENEW] = e;
-            scal[S_BETA] = e / scal[S_EOLD];
+            scal[S_CONV] = fabs(scal[S_EOLD]) > scal[S_TOL] ? 0.0 : 1.0;
         }
-        *counter = 0;
+        return;
+    }
+    double e = combine_partials(partials, nb, 0.0, OpSum());
+    if (threadIdx.x == 0) {
+        scal[S_K] = scal[S_K] + 1.0;
+        scal[S_ENEW] = e;
+        scal[S_BETA] = e / scal[S_EOLD];
     }
 }
 
@@ -141,12 +154,13 @@ __global__ void __launch_bounds__(kBlock)
 k_iter_b(int n, const double* __restrict__ A, const int* __restrict__ idx,
          const int* __restrict__ rowptr, const int* __restrict__ fixflag,
          const double* __restrict__ pold, const double* __restrict__ z, double* __restrict__ pnew,
-         double* __restrict__ y, int first, double* __restrict__ scal,
-         double* __restrict__ partials, unsigned* __restrict__ counter) {
+         double* __restrict__ y, int first, const double* __restrict__ scal,
+         double* __restrict__ partials) {
     if (scal[S_DONE] != 0.0 || scal[S_STOP] != 0.0) return;
     const double beta = scal[S_BETA];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
-    GRID_STRIDE(i, n) {
+    if (i < n) {
         double pi = first ? z[i] : beta * pold[i] + z[i];
         double yi;
         if (fixflag[i]) {
@@ -162,10 +176,15 @@ k_iter_b(int n, const double* __restrict__ A, const int* __restrict__ idx,
         }
         pnew[i] = pi;
         y[i] = yi;
-        v = v + pi * yi;
+        v = pi * yi;
     }
-    if (!block_partial(v, OpSum(), partials, counter)) return;
-    double py = combine_partials(partials, gridDim.x, 0.0, OpSum());
+    block_partial(v, OpSum(), partials);
+}
+
+__global__ void __launch_bounds__(256)
+k_fin_b(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
+    if (scal[S_DONE] != 0.0 || scal[S_STOP] != 0.0) return;
+    double py = combine_partials(partials, nb, 0.0, OpSum());
     if (threadIdx.x == 0) {
         double e = scal[S_ENEW];
         scal[S_PY] = py;
@@ -173,7 +192,6 @@ k_iter_b(int n, const double* __restrict__ A, const int* __restrict__ idx,
         scal[S_EOLD] = e;
         bool more = fabs(e) > scal[S_TOL] && scal[S_K] < scal[S_MAXIT];
         if (!more) scal[S_STOP] = 1.0;
-        *counter = 0;
     }
 }
 
@@ -182,7 +200,7 @@ k_iter_b(int n, const double* __restrict__ A, const int* __restrict__ idx,
 void PcgWork::ensure(int npoin) {
     if (npoin == n && r.p) return;
     n = npoin;
-    nblocks = (int)red_grid(npoin);
+    nblocks = (int)grid_for(npoin);
     r.alloc(npoin);
     z.alloc(npoin);
     y.alloc(npoin);
@@ -191,8 +209,6 @@ void PcgWork::ensure(int npoin) {
     fixflag.alloc(npoin);
     partial.alloc(nblocks);
     scal.alloc(16);
-    counter.alloc(1);
-    CFD_HIP(hipMemset(counter.p, 0, sizeof(unsigned)));
 }
 
 PcgResult pcg_solve(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
@@ -200,40 +216,47 @@ PcgResult pcg_solve(PcgWork& w, int npoin, const double* A, const int* idx, cons
                     const int* x_fix_idx, int nfix, double tol, int maxit, int check_every,
                     hipStream_t s) {
     w.ensure(npoin);
-    const dim3 g(red_grid(npoin)), blk(kBlock);
+    const int nb = w.nblocks;
+    const dim3 g(nb), blk(kBlock), one(1), fin(256);
     double* scal = w.scal.p;
+    double* part = w.partial.p;
     double h[16];
     auto poll = [&]() {
         CFD_HIP(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, s));
         CFD_HIP(hipStreamSynchronize(s));
     };
-    k_scal_init<<<1, 1, 0, s>>>(scal, tol, maxit);
+    double* pbuf[2] = {w.pa.p, w.pb.p};
+    auto iter_a = [&](int cur) {
+        k_iter_a<<<g, blk, 0, s>>>(npoin, x, pbuf[cur], w.r.p, w.y.p, diag, w.z.p, scal, part);
+        k_fin_a<<<one, fin, 0, s>>>(part, nb, scal);
+    };
+    auto iter_b = [&](int cur, int first) {
+        k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[cur], w.z.p,
+                                   pbuf[cur ^ 1], w.y.p, first, scal, part);
+        k_fin_b<<<one, fin, 0, s>>>(part, nb, scal);
+    };
+    k_scal_init<<<one, one, 0, s>>>(scal, tol, maxit);
     k_fix_init<<<g, blk, 0, s>>>(npoin, w.fixflag.p);
     if (nfix > 0)
         k_fix_set<<<grid_for(nfix), blk, 0, s>>>(nfix, x_fix_idx, x_fix, w.fixflag.p, x);
-    k_residual<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, x, b, w.r.p, scal,
-                                 w.partial.p, w.counter.p);
-    k_first_z<<<g, blk, 0, s>>>(npoin, w.r.p, diag, w.z.p, scal, w.partial.p, w.counter.p);
-    double* pbuf[2] = {w.pa.p, w.pb.p};
-    k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[1], w.z.p, pbuf[0], w.y.p, 1,
-                               scal, w.partial.p, w.counter.p);
+    k_residual<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, x, b, w.r.p, part);
+    k_fin_residual<<<one, fin, 0, s>>>(part, nb, scal);
+    k_first_z<<<g, blk, 0, s>>>(npoin, w.r.p, diag, w.z.p, scal, part);
+    k_fin_first_z<<<one, fin, 0, s>>>(part, nb, scal);
+    iter_b(1, 1);  // p0 = z0 into pbuf[0]
     CFD_HIP(hipGetLastError());
-    poll();  // most warm-started mesh solves stop here (r.r < tol)
-    int cur = 0;  // pbuf[cur] holds the current p
+    poll();        // most warm-started mesh solves stop here (r.r < tol)
+    int cur = 0;   // pbuf[cur] holds the current p
     const int chunk = check_every > 0 ? check_every : 8;
     for (int launched = 0; h[S_DONE] == 0.0; launched += chunk) {
-        if (h[S_STOP] != 0.0) {
-            // the loop test failed: one A applies the final x update and ends the solve
-            k_iter_a<<<g, blk, 0, s>>>(npoin, x, pbuf[cur], w.r.p, w.y.p, diag, w.z.p, scal,
-                                       w.partial.p, w.counter.p);
+        if (h[S_STOP] != 0.0) {  // one A applies the final x update and ends the solve
+            iter_a(cur);
             poll();
             break;
         }
         for (int q = 0; q < chunk; ++q) {
-            k_iter_a<<<g, blk, 0, s>>>(npoin, x, pbuf[cur], w.r.p, w.y.p, diag, w.z.p, scal,
-                                       w.partial.p, w.counter.p);
-            k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[cur], w.z.p,
-                                       pbuf[cur ^ 1], w.y.p, 0, scal, w.partial.p, w.counter.p);
+            iter_a(cur);
+            iter_b(cur, 0);
             cur ^= 1;
         }
         CFD_HIP(hipGetLastError());

@@ -1,22 +1,13 @@
-// Deterministic grid reductions: every block reduces its values in a fixed
-// tree, stores one partial, and the last block to arrive (atomic ticket)
-// combines the partials in index order.  The result does not depend on block
-// scheduling, so repeated runs are bitwise identical.
+// Deterministic grid reductions without atomics: a full-grid kernel reduces
+// each block's values in a fixed tree and stores one partial per block; a
+// one-block finalize kernel combines the partials, each thread folding a fixed
+// strided subset in index order, then a fixed tree.  The result does not
+// depend on scheduling, so repeated runs are bitwise identical.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace cfd {
-
-// Reducing kernels run a fixed grid of at most kRedGrid blocks with a
-// grid-stride loop: few partials (one ticket atomic per block on a single
-// address is the expensive part), and a partition that depends on n only.
-constexpr unsigned kRedGrid = 512;
-inline unsigned red_grid(size_t n, unsigned block = 256) {
-    size_t g = (n + block - 1) / block;
-    if (g == 0) g = 1;
-    return g < kRedGrid ? (unsigned)g : kRedGrid;
-}
 
 template <class Op>
 __device__ inline double wave_reduce(double v, Op op) {
@@ -31,48 +22,46 @@ struct OpSum {
     __device__ double operator()(double a, double b) const { return a + b; }
 };
 
-// Reduces `v` over the block into partials[blockIdx.x]; returns true in every
-// thread of the last block (after an acquire fence), so that block can read
-// all partials.  Block size must be a multiple of 64 and at most 1024.
+// Block-wide reduction; the value is valid in thread 0.  Every thread of the
+// block must call it.  Block size: a multiple of 64, at most 1024.
 template <class Op>
-__device__ inline bool block_partial(double v, Op op, double* partials, unsigned* counter) {
+__device__ inline double block_reduce(double v, Op op) {
     __shared__ double sh[16];
-    __shared__ int last;
     v = wave_reduce(v, op);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();  // sh may still be read by a previous call
     if (lane == 0) sh[wave] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double acc = sh[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) acc = op(acc, sh[w]);
-        partials[blockIdx.x] = acc;
-        __threadfence();
-        unsigned ticket = atomicAdd(counter, 1u);
-        last = (ticket == gridDim.x - 1);
+        v = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = op(v, sh[w]);
     }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
+    return v;
 }
 
-// In the last block, called by every thread: combine partials[0..n).
-// Thread t folds partials t, t+blockDim, ... in index order, then the block
-// tree-reduces; the assignment and the tree are fixed, so the result is the
-// same on every run.  The value is valid in thread 0.
+// Full-grid pass: one partial per block.
+template <class Op>
+__device__ inline void block_partial(double v, Op op, double* partials) {
+    v = block_reduce(v, op);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+// One-block finalize: combine partials[0..n); valid in thread 0.
 template <class Op>
 __device__ inline double combine_partials(const double* partials, int n, double init, Op op) {
-    __shared__ double shc[16];
     double acc = init;
     for (int b = threadIdx.x; b < n; b += blockDim.x) acc = op(acc, partials[b]);
-    acc = wave_reduce(acc, op);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) shc[wave] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        acc = shc[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) acc = op(acc, shc[w]);
-    }
-    return acc;
+    return block_reduce(acc, op);
 }
+
+namespace {
+// *out = min(init, partials[0..n)), then min(*out, cap) (one block of 256)
+__global__ void __launch_bounds__(256)
+k_min_finalize(const double* __restrict__ partials, int n, double init, double cap,
+               double* __restrict__ out) {
+    double m = combine_partials(partials, n, init, OpMin());
+    if (threadIdx.x == 0) *out = m > cap ? cap : m;
+}
+}  // namespace
 
 }  // namespace cfd

@@ -311,3 +311,43 @@ def test_estab_bitwise_fraction(cyl, mctx):
     fr = {k: 1.0 - bitdiff(a, s[k]) / a.size for a, k in zip(out, ("SHOC", "T_SUGN1", "T_SUGN2", "T_SUGN3"))}
     print("ESTAB bitwise-identical fraction:", fr)
     assert fr["SHOC"] == 1.0      # no pow on this path
+
+
+# ---------------------------------------------------------------------------
+# FP32-element / FP64-accumulation path
+# ---------------------------------------------------------------------------
+def test_fp32_element_step_against_fp64_oracle(cyl):
+    """One teacher-forced step: the FP32 element residual (FP64 gather and
+    update) against the FP64 oracle; binary32 arithmetic bounds the error."""
+    from cfd_amd.solver import Session
+    cs, o0 = cyl
+    o = OracleSolver(cs)
+    o.set_state(o0.state())
+    S = Session(cs, precision=1)
+    S.set_state(o.state())
+    o.step()
+    S.step(1)
+    ref, got = o.state(), S.state()
+    rel_rhs = relnorm_cols(got["RHS"], ref["RHS"])
+    print("FP32 element: RHS rel err per component", rel_rhs)
+    assert np.all(rel_rhs < 2e-4), rel_rhs
+    assert np.all(relnorm_cols(got["U"], ref["U"]) < 1e-7)
+    assert got["DTMIN"] == ref["DTMIN"]
+
+
+def test_fp32_free_run_density_and_forces(cyl):
+    """Free-running FP32-element vs FP64 paths on the GPU: L2 density error
+    and pressure force after 200 steps."""
+    from cfd_amd.solver import Session
+    cs, _ = cyl
+    a = Session(cs, precision=0)
+    b = Session(cs, precision=1)
+    a.step(200)
+    b.step(200)
+    ra, rb = a.get("U")[:, 0], b.get("U")[:, 0]
+    l2 = relnorm(rb, ra)
+    sa, sb = a.scalars(), b.scalars()
+    dfx = abs(sb["FX1"] - sa["FX1"]) / abs(sa["FX1"])
+    print(f"FP32 vs FP64 after 200 steps: L2(rho) {l2:.3e}, Fx rel {dfx:.3e}")
+    assert np.all(np.isfinite(b.get("U")))
+    assert l2 < 1e-3 and dfx < 1e-2
