@@ -28,11 +28,13 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "triangle-RHS evals/s, %HBM roofline; ALE step time learned vs plain at equal tol"
 
 
-def element_bytes(nelem: int, npoin: int) -> int:
-    """Algorithmic bytes of one element-kernel launch (calcRHS element part):
-    per element inpoel 12 + dNx,dNy 48 + area,dtl,shoc,tau1..3 48 = 108 B; per
-    node U 32 + T 8 = 40 B (theta == 0 inside RK is not read)."""
-    return 108 * nelem + 40 * npoin
+def element_bytes(nelem: int, npoin: int, precision: int = 0) -> int:
+    """Algorithmic bytes of one element-kernel launch (calcRHS element part),
+    each input counted once at the width the variant reads:
+      FP64: per element inpoel 12 + dNx,dNy 48 + area,dtl,shoc,tau1..3 48 = 108 B
+      FP32: per element inpoel 12 + dNx,dNy 24 + fac,nu,tau1..3 20 = 56 B
+      per node U 32 + T 8 = 40 B (theta == 0 inside RK is not read)."""
+    return (108 if precision == 0 else 56) * nelem + 40 * npoin
 
 
 def cpu_baseline(cs, seconds_budget: float = 20.0) -> dict:
@@ -92,7 +94,8 @@ def main():
     S = Session(cs, precision=prec)
     S.step(args.warmup)
     S.sync()
-    S.timing(True)
+    # timed region: HIP events only around the element kernel launches
+    S.timing(True, slots=["ELEMENT"])
     S.sync()
     t0 = time.perf_counter()
     S.step(args.steps)
@@ -101,16 +104,20 @@ def main():
     sec = t1 - t0
     value = 4.0 * cs.nelem * args.steps / sec
     te, ne = S.kernel_time("ELEMENT")
-    ts, ns = S.kernel_time("STAGE")
     avg_elem_s = te / ne * 1e-3
-    bytes_elem = element_bytes(cs.nelem, cs.npoin)
+    bytes_elem = element_bytes(cs.nelem, cs.npoin, prec)
     achieved = bytes_elem / avg_elem_s / 1e9
+    sc = S.scalars()
+    # per-phase breakdown: a separate, untimed pass with events on every phase
+    S.timing(True)
+    S.step(min(args.steps, 10))
+    S.sync()
     kern = {}
     for k in ("ELEMENT", "STAGE", "ESTAB", "DELTAT", "PCG", "GEOM", "STEP"):
         t, n = S.kernel_time(k)
         if n:
             kern[k.lower()] = {"avg_us": round(t / n * 1e3, 2), "launches": n}
-    sc = S.scalars()
+    S.timing(False)
     out = {
         "metric": METRIC,
         "value": value,

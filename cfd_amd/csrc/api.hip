@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "cfd_internal.hpp"
+#include "reduce.hpp"
 
 namespace cfd {
 
@@ -28,16 +29,6 @@ static Consts to_consts(const cfd_consts* c) {
 // small kernels used by the entries and the session
 // ---------------------------------------------------------------------------
 namespace {
-__global__ void k_fill(double* __restrict__ a, double v, long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-// DTL = DTMIN*f + DT*(1-f) (ns2DComp.ALE.f90:160-161)
-__global__ void k_dtl_blend(double* __restrict__ dtl, const double* __restrict__ DT, double dtmin,
-                            double f, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dtl[i] = dtmin * f + DT[i] * (1.0 - f);
-}
 // free stream (ns2DComp.ALE.f90:409-420) plus the start values of the work arrays
 __global__ void k_freestream(double* __restrict__ U, double* __restrict__ U1, double* __restrict__ VX,
                              double* __restrict__ VY, double* __restrict__ T, double* __restrict__ GAMM,
@@ -69,13 +60,84 @@ __global__ void k_pos_aux(const int* __restrict__ ilaux, const double* __restric
 }
 // X += XPOS; X1 += XPOS; W = XPOS/DTMIN (meshMove.f90:99-104)
 __global__ void k_move(double* __restrict__ X, double* __restrict__ X1, double* __restrict__ W,
-                       const double* __restrict__ pos, double dtmin, int n) {
+                       const double* __restrict__ pos, const double* __restrict__ dtminp, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double dtmin = *dtminp;
     double d = pos[i];
     X[i] = X[i] + d;
     X1[i] = X1[i] + d;
     W[i] = d / dtmin;
+}
+
+// Session scalars live on the device: the CFD_S_* slots, then work slots.
+enum { X_RAW = CFD_NSCALARS, X_FDTL, X_CA, X_SA, X_NCTL };
+
+// Step control (ns2DComp.ALE.f90:142-166, meshMove.f90:55-70): DTMIN from
+// the deltat partials, the 1 % hysteresis against DTMIN1, the clock, the
+// local-time-step blend factor and the body's rotation increment for TRANSF.
+// BANDERA's increment after fluidStructure (ns2DComp.ALE.f90:250) is applied
+// here too: nothing in between reads it.
+__global__ void __launch_bounds__(256)
+k_step_ctl(const double* __restrict__ partials, int nb, double* __restrict__ ctl, int itlocal,
+           int motion, double amp, double omega) {
+    const double m = combine_partials(partials, nb, 1.0e20, OpMin());
+    if (threadIdx.x != 0) return;
+    double dtmin = m;
+    ctl[X_RAW] = dtmin;
+    const double iter = ctl[CFD_S_ITER] + 1.0;
+    ctl[CFD_S_ITER] = iter;
+    double bandera = ctl[CFD_S_BANDERA];
+    double dtmin1 = ctl[CFD_S_DTMIN1];
+    if ((int)bandera == 1) {
+        dtmin1 = dtmin;
+        bandera = 2.0;
+    }
+    const double porc = fabs((dtmin - dtmin1) / dtmin);
+    if (100.0 * porc <= 1.0) {
+        dtmin = dtmin1;
+    } else {
+        dtmin1 = dtmin;
+        bandera = 2.0;
+    }
+    ctl[CFD_S_DTMIN1] = dtmin1;
+    ctl[CFD_S_BANDERA] = bandera + 1.0;
+    ctl[CFD_S_DTMIN] = dtmin;
+    ctl[X_FDTL] = itlocal != 0 ? 1.0 - exp(-iter * 4.6 / itlocal) : 1.0;
+    const double time = ctl[CFD_S_TIME] + dtmin;
+    ctl[CFD_S_TIME] = time;
+    // meshMove.f90:55-57,70: ampli*sin(10 t), ampli = pi/8 (motion 0)
+    const double pi = 3.141592653589793;
+    const double ang = motion == 1 ? amp * sin(omega * time) : pi / 8.0 * sin(10.0 * time);
+    const double alpha = ang - ctl[CFD_S_DISN2];
+    ctl[CFD_S_DISN2] = ang;
+    ctl[X_CA] = cos(alpha);
+    ctl[X_SA] = sin(alpha);
+}
+
+// DT clipped at 10*min(DT) (subrutinas.f90:210-216), then DTL = DTMIN, or
+// DTMIN*f + DT*(1-f) with local time stepping (ns2DComp.ALE.f90:158-164)
+__global__ void k_dt_finish(double* __restrict__ DT, double* __restrict__ DTL,
+                            const double* __restrict__ ctl, int blend, int n) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double cota = 10.0 * ctl[X_RAW], dtmin = ctl[CFD_S_DTMIN];
+    double d = DT[e];
+    if (d > cota) {
+        d = cota;
+        DT[e] = d;
+    }
+    if (blend) {
+        const double f = ctl[X_FDTL];
+        DTL[e] = dtmin * f + d * (1.0 - f);
+    } else {
+        DTL[e] = dtmin;
+    }
+}
+
+__global__ void k_put2(double* __restrict__ a, double va, double* __restrict__ b, double vb) {
+    *a = va;
+    *b = vb;
 }
 }  // namespace
 
@@ -383,7 +445,8 @@ int cfd_estab(cfd_mesh* m, const cfd_consts* c, const double* U, const double* T
         double* o1 = m->scratch("t1", ne);
         double* o2 = m->scratch("t2", ne);
         double* o3 = m->scratch("t3", ne);
-        launch_estab(m->t, k, u, t, g, vx, vy, wx, wy, nx, ny, dtmin, o0, o1, o2, o3, m->s);
+        const double* dtm = m->put("dtmin", &dtmin, 1);
+        launch_estab(m->t, k, u, t, g, vx, vy, wx, wy, nx, ny, dtm, o0, o1, o2, o3, m->s);
         m->get(shoc, o0, ne);
         m->get(t1, o1, ne);
         m->get(t2, o2, ne);
@@ -492,13 +555,14 @@ int cfd_rk(cfd_mesh* m, const cfd_consts* c, const cfd_bc_lists* bc, int nrk, do
         double* cc = m->scratch("c_calc", 12 * (size_t)ne);
         double* cf = m->scratch("c_fuente", 12 * (size_t)ne);
         // ESTAB at stage 1 with the start-of-step U and T (subrutinas.f90:671-677)
-        launch_estab(m->t, k, U, T, G, VX, VY, WX, WY, nx, ny, dtmin, sh, t1, t2, t3, m->s);
+        const double* dtm = m->put("dtmin", &dtmin, 1);
+        launch_estab(m->t, k, U, T, G, VX, VY, WX, WY, nx, ny, dtm, sh, t1, t2, t3, m->s);
         ElemIn in{U, nullptr, T, nx, ny, ar, sh, dl, t1, t2, t3, WX, WY};
         StageNodes sn{U, U1, RHS, T, VX, VY, RHO, E, P, RM, M, G, WX, WY};
         for (int irk = 1; irk <= nrk; ++irk) {
             double rk_fact = 1.0 / (nrk + 1 - irk);
             launch_element_rhs(m->t, k, in, cc, cf, m->s);
-            launch_stage_update(m->t, k, m->bc->tab, rk_fact, cc, cf, sn, true, m->s);
+            launch_stage_update(m->t, k, m->bc->tab, rk_fact, cc, cf, sn, irk == nrk, m->s);
         }
         m->get(nd->U1, (const double*)U1, n4);
         m->get(nd->RHS, (const double*)RHS, n4);
@@ -588,7 +652,8 @@ int cfd_gcl_main(cfd_mesh* m, double* M, const double* W_x, const double* W_y, c
         double* nx = m->put("dNx", dNx, 3 * (size_t)ne);
         double* ny = m->put("dNy", dNy, 3 * (size_t)ne);
         double* a = m->put("area", area, ne);
-        launch_gcl(m->t, dM, wx, wy, m->w_x_old.p, m->w_y_old.p, nx, ny, a, m->area_old.p, dt,
+        const double* dtp = m->put("dt", &dt, 1);
+        launch_gcl(m->t, dM, wx, wy, m->w_x_old.p, m->w_y_old.p, nx, ny, a, m->area_old.p, dtp,
                    mode ? 1 : 0, m->s);
         m->get(M, (const double*)dM, np);
         m->sync();
@@ -601,14 +666,15 @@ int cfd_gcl_main(cfd_mesh* m, double* M, const double* W_x, const double* W_y, c
 // Device-resident session: the main program's time step
 // (ns2DComp.ALE.f90:140-281) with every array in HBM.
 //
-// Per step: deltat -> DTMIN (one 8-byte read-back for the host-side
-// hysteresis and clock, :144-166) -> RK (ESTAB, then nrk x [element
-// residual, gather + stage update]) -> fluidStructure (FORCES, TRANSF, two
-// PCG mesh solves, X += XPOS, W = XPOS/DTMIN) -> on moving meshes NORMALES,
-// DERIV, MASAS (or the GCL update), laplace -> U = U1 (a buffer swap).
+// Per step: deltat -> DTMIN, its hysteresis, DTL and the clock on the device
+// (:142-166) -> RK (ESTAB, then nrk x [element residual, gather + stage
+// update]) -> fluidStructure (FORCES, TRANSF, two PCG mesh solves, X += XPOS,
+// W = XPOS/DTMIN) -> on moving meshes NORMALES, DERIV, MASAS (or the GCL
+// update), laplace -> U = U1 (a buffer swap).  All scalars stay in device
+// memory (ctl), so a step is queued without waiting for the device.
 //
 // The reference copies U1 = U before RK (:168-172); RK overwrites all of U1
-// in its first stage before anything reads it, so no copy is made here.
+// in its last stage before anything reads it, so no copy is made here.
 // FUENTE adds exactly +-0 while the mesh velocity is identically zero
 // (fixed mesh, no moving nodes, W never set by the caller); it is skipped then.
 // ---------------------------------------------------------------------------
@@ -623,17 +689,23 @@ struct cfd_session {
     int ucur = 0;
     DBuf<double> RHS, T, VX, VY, WX, WY, X, Y, X1, Y1, RHO, E, P, RMACH, M, GAMM, XPOS, YPOS;
     DBuf<double> DTL, DT, SHOC, T1, T2, T3, AREA, DNX, DNY, HH, HHX, HHY, AREA_OLD, WXO, WYO;
-    DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, BZERO, dscal, partials;
+    DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, BZERO, partials;
     // FP32 element path: binary32 shape gradients, ESTAB outputs, slots
     DBuf<float> DNX32, DNY32, NU32, TA32, TB32, TC32, FAC32, c32, f32b;
     DBuf<int> ILAUX, ISET;
     BcDevice bc;
     PcgWork pcg;
-    double sc[CFD_NSCALARS] = {0};
-    double* hpin = nullptr;
+    DBuf<double> ctl;                  // session scalars on the device (X_NCTL slots)
+    double sc[CFD_NSCALARS] = {0};     // host copy, refreshed by sync_scalars()
     bool w_dirty = false;
+    // The mesh solves of a mesh without moving nodes return at their first
+    // residual test while XPOS/YPOS are zero (the right-hand side and every
+    // fixed value are zero): XPOS/YPOS stay zero and X, Y, W do not change
+    // once one X += 0, W = 0/DTMIN pass has normalised signed zeros.
+    bool xpos_zero = true, move_normalized = false;
     // timing
     bool timing = false;
+    unsigned tmask = 0;
     double tacc[CFD_NKTIMERS] = {0};
     long long tcnt[CFD_NKTIMERS] = {0};
     std::vector<hipEvent_t> pool;
@@ -654,12 +726,12 @@ struct cfd_session {
         return e;
     }
     void tb(int slot) {
-        if (!timing) return;
+        if (!timing || !((tmask >> slot) & 1u)) return;
         open_ev[slot] = ev();
         CFD_HIP(hipEventRecord(open_ev[slot], s));
     }
     void te(int slot) {
-        if (!timing) return;
+        if (!timing || !((tmask >> slot) & 1u)) return;
         hipEvent_t b = ev();
         CFD_HIP(hipEventRecord(b, s));
         pending.push_back({open_ev[slot], b, slot});
@@ -680,15 +752,23 @@ struct cfd_session {
         if (s) (void)hipStreamSynchronize(s);
         for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
         for (auto e : pool) (void)hipEventDestroy(e);
-        if (hpin) (void)hipHostFree(hpin);
     }
     bool fuente_needed() const { return d.moving || d.nmove > 0 || w_dirty; }
+    double* scal(int k) { return ctl.p + k; }
+    void sync_scalars() {  // device scalars -> sc (waits for the stream)
+        ctl.download(sc, CFD_NSCALARS, s);
+        CFD_HIP(hipStreamSynchronize(s));
+    }
+    void put_scalars() {  // sc -> device scalars
+        ctl.upload(sc, CFD_NSCALARS, s);
+        CFD_HIP(hipStreamSynchronize(s));
+    }
 
     void geometry() {  // NORMALES, DERIV, MASAS, laplace (ns2DComp.ALE.f90:88-100, 259-275)
         const Tables& t = m->t;
         bc.normales(X.p, Y.p, s);
-        launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p, dscal.p + 1, partials.p,
-                     s);
+        launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p, scal(CFD_S_HMIN),
+                     partials.p, s);
         refresh_f32_geometry();
         launch_masas(t, AREA.p, M.p, s);
         launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
@@ -701,60 +781,41 @@ struct cfd_session {
         launch_to_f32(DNY.p, DNY32.p, 3LL * ne, s);
     }
 
-    double motion_angle(double time) const {
-        if (d.motion == 1) return d.motion_amp * std::sin(d.motion_omega * time);
-        const double pi = std::acos(-1.0), ampli = pi / 8.0;  // meshMove.f90:55-57,70
-        return ampli * std::sin(10.0 * time);
-    }
-
+    // One biCG mesh solve of fluidStructure (meshMove.f90:91-104), then
+    // X += XPOS, X1 += XPOS, W = XPOS/DTMIN.  The host waits for the solve's
+    // convergence test.
     void mesh_solve(DBuf<double>& dpos, DBuf<double>& pos, DBuf<double>& coord, DBuf<double>& coord1,
-                    DBuf<double>& w, double dtmin, int which) {
+                    DBuf<double>& w, int which) {
         const int nn = d.nmove + d.nfix_move;
         if (nn > 0)
             k_pos_aux<<<grid_for(nn), kBlock, 0, s>>>(ILAUX.p, dpos.p, POSAUX.p, d.nmove, nn);
         tb(CFD_K_PCG);
         double tol = d.pcg_tol > 0.0 ? d.pcg_tol : 1.0e-10;
         int maxit = d.pcg_maxit > 0 ? d.pcg_maxit : 1000;
-        PcgResult r = pcg_solve(pcg, np, LAP.p, m->t.idx.p, m->t.rowptr.p, LDIAG.p, pos.p, BZERO.p,
-                                POSAUX.p, ILAUX.p, nn, tol, maxit, 8, s);
+        pcg_solve(pcg, np, LAP.p, m->t.idx.p, m->t.rowptr.p, LDIAG.p, pos.p, BZERO.p, POSAUX.p,
+                  ILAUX.p, nn, tol, maxit, 8, s);
         te(CFD_K_PCG);
-        sc[which ? CFD_S_PCG_ITY : CFD_S_PCG_ITX] = r.iters;
-        sc[which ? CFD_S_PCG_RZY : CFD_S_PCG_RZX] = r.rz;
-        k_move<<<grid_for(np), kBlock, 0, s>>>(coord.p, coord1.p, w.p, pos.p, dtmin, np);
+        pcg_store_result(pcg, scal(which ? CFD_S_PCG_ITY : CFD_S_PCG_ITX),
+                         scal(which ? CFD_S_PCG_RZY : CFD_S_PCG_RZX), s);
+        k_move<<<grid_for(np), kBlock, 0, s>>>(coord.p, coord1.p, w.p, pos.p, scal(CFD_S_DTMIN), np);
         CFD_HIP(hipGetLastError());
     }
 
+    // One time step, queued on the stream without waiting for the device
+    // (unless the mesh moves: the mesh solves test convergence on the host).
     void step() {
         const Tables& t = m->t;
         tb(CFD_K_STEP);
-        sc[CFD_S_ITER] += 1.0;
-        // deltat and the DTMIN hysteresis (ns2DComp.ALE.f90:144-166)
+        // deltat, DTMIN with its hysteresis, DTL, the clock (ns2DComp.ALE.f90:142-166)
         tb(CFD_K_DELTAT);
-        launch_deltat(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p, DT.p, dscal.p, partials.p, s);
+        const int nb = launch_deltat_partials(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p, DT.p,
+                                              partials.p, s);
+        k_step_ctl<<<1, 256, 0, s>>>(partials.p, nb, ctl.p, d.itlocal, d.motion, d.motion_amp,
+                                     d.motion_omega);
+        k_dt_finish<<<grid_for(ne), kBlock, 0, s>>>(DT.p, DTL.p, ctl.p, d.itlocal != 0, ne);
+        CFD_HIP(hipGetLastError());
         te(CFD_K_DELTAT);
-        CFD_HIP(hipMemcpyAsync(hpin, dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
-        CFD_HIP(hipStreamSynchronize(s));
-        double dtmin = hpin[0];
-        if ((int)sc[CFD_S_BANDERA] == 1) {
-            sc[CFD_S_DTMIN1] = dtmin;
-            sc[CFD_S_BANDERA] = 2;
-        }
-        double porc = std::fabs((dtmin - sc[CFD_S_DTMIN1]) / dtmin);
-        if (100.0 * porc <= 1.0) {
-            dtmin = sc[CFD_S_DTMIN1];
-        } else {
-            sc[CFD_S_DTMIN1] = dtmin;
-            sc[CFD_S_BANDERA] = 2;
-        }
-        if (d.itlocal != 0) {
-            double f = 1.0 - std::exp(-sc[CFD_S_ITER] * 4.6 / d.itlocal);
-            k_dtl_blend<<<grid_for(ne), kBlock, 0, s>>>(DTL.p, DT.p, dtmin, f, ne);
-        } else {
-            k_fill<<<grid_for(ne), kBlock, 0, s>>>(DTL.p, dtmin, ne);
-        }
-        sc[CFD_S_DTMIN] = dtmin;
-        sc[CFD_S_TIME] = sc[CFD_S_TIME] + dtmin;
-        const double time = sc[CFD_S_TIME];
+        const double* dtmin = scal(CFD_S_DTMIN);
 
         // RK (subrutinas.f90:645-849)
         const bool f32 = d.precision == 1;
@@ -804,16 +865,21 @@ struct cfd_session {
             CFD_HIP(hipMemcpyAsync(WXO.p, WX.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
             CFD_HIP(hipMemcpyAsync(WYO.p, WY.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
         }
-        if (d.nsetedges > 0)
-            launch_forces(d.nsetedges, ISET.p, X.p, Y.p, P.p, d.xref, d.yref, dscal.p + 2, s);
-        double alphav = sc[CFD_S_DISN2];
-        sc[CFD_S_DISN2] = motion_angle(time);
-        double alpha = sc[CFD_S_DISN2] - alphav, yposr = 0.0;
-        launch_transf(d.nsetedges, ISET.p, X.p, Y.p, std::cos(alpha), std::sin(alpha), yposr, d.xref,
-                      d.yref, DXPOS.p, DYPOS.p, s);
-        mesh_solve(DXPOS, XPOS, X, X1, WX, dtmin, 0);
-        mesh_solve(DYPOS, YPOS, Y, Y1, WY, dtmin, 1);
-        sc[CFD_S_BANDERA] += 1.0;
+        launch_forces(d.nsetedges, ISET.p, X.p, Y.p, P.p, d.xref, d.yref, scal(CFD_S_FX), s);
+        if (d.nmove == 0 && xpos_zero && move_normalized) {
+            // both solves return at once: nothing moves (see xpos_zero)
+            k_put2<<<1, 1, 0, s>>>(scal(CFD_S_PCG_ITX), 0.0, scal(CFD_S_PCG_RZX), 0.0);
+            k_put2<<<1, 1, 0, s>>>(scal(CFD_S_PCG_ITY), 0.0, scal(CFD_S_PCG_RZY), 0.0);
+            CFD_HIP(hipGetLastError());
+        } else {
+            // TRANSF's displacements reach the solves only through moving nodes
+            if (d.nmove > 0)
+                launch_transf(d.nsetedges, ISET.p, X.p, Y.p, scal(X_CA), 0.0, d.xref, d.yref,
+                              DXPOS.p, DYPOS.p, s);
+            mesh_solve(DXPOS, XPOS, X, X1, WX, 0);
+            mesh_solve(DYPOS, YPOS, Y, Y1, WY, 1);
+            move_normalized = true;
+        }
 
         if (d.moving == 1) {  // ns2DComp.ALE.f90:259-275
             tb(CFD_K_GEOM);
@@ -821,8 +887,8 @@ struct cfd_session {
                 CFD_HIP(hipMemcpyAsync(AREA_OLD.p, AREA.p, sizeof(double) * ne,
                                        hipMemcpyDeviceToDevice, s));
                 bc.normales(X.p, Y.p, s);
-                launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p, dscal.p + 1,
-                             partials.p, s);
+                launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p,
+                             scal(CFD_S_HMIN), partials.p, s);
                 refresh_f32_geometry();
                 launch_gcl(t, M.p, WX.p, WY.p, WXO.p, WYO.p, DNX.p, DNY.p, AREA.p, AREA_OLD.p, dtmin,
                            d.gcl, s);
@@ -834,13 +900,6 @@ struct cfd_session {
         }
         ucur ^= 1;  // U = U1 (:277-281)
         te(CFD_K_STEP);
-        CFD_HIP(hipMemcpyAsync(hpin + 1, dscal.p + 1, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-        CFD_HIP(hipStreamSynchronize(s));
-        sc[CFD_S_HMIN] = hpin[1];
-        sc[CFD_S_FX] = hpin[2];
-        sc[CFD_S_FY] = hpin[3];
-        sc[CFD_S_RM] = hpin[4];
-        resolve();
     }
 
     DBuf<double>* field(int f, long long& count) {
@@ -941,8 +1000,8 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->ILAUX.upload(S->ilaux_h.data(), nn, s);
         S->ISET.alloc(std::max<size_t>(3, S->iset_h.size()));
         S->ISET.upload(S->iset_h.data(), S->iset_h.size(), s);
-        S->dscal.alloc(8);
-        S->dscal.zero(s);
+        S->ctl.alloc(X_NCTL);
+        S->ctl.zero(s);
         S->partials.alloc(grid_for(std::max(ne, np)));
         // zero everything once (DXPOS/DYPOS keep zeros off the set nodes,
         // meshMove.f90:52-53; B stays zero, :91-95)
@@ -957,9 +1016,8 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         BcHost h = build_bc_host(np, &d->bc);
         S->bc.upload(h, np, &d->bc, s);
         S->pcg.ensure(np);
-        CFD_HIP(hipHostMalloc(&S->hpin, 16 * sizeof(double)));
         S->sc[CFD_S_BANDERA] = 1.0;
-        CFD_HIP(hipStreamSynchronize(s));
+        S->put_scalars();
         *out = S.release();
     })
 }
@@ -982,12 +1040,12 @@ int cfd_session_init_freestream(cfd_session* S) {
         S->geometry();
         CFD_HIP(hipMemcpyAsync(S->AREA_OLD.p, S->AREA.p, sizeof(double) * S->ne,
                                hipMemcpyDeviceToDevice, S->s));
-        CFD_HIP(hipMemcpyAsync(S->hpin + 1, S->dscal.p + 1, sizeof(double), hipMemcpyDeviceToHost, S->s));
-        CFD_HIP(hipStreamSynchronize(S->s));
-        S->sc[CFD_S_HMIN] = S->hpin[1];
+        S->sync_scalars();
         for (int k : {CFD_S_TIME, CFD_S_ITER, CFD_S_DTMIN, CFD_S_DTMIN1, CFD_S_DISN2}) S->sc[k] = 0.0;
         S->sc[CFD_S_BANDERA] = 1.0;
+        S->put_scalars();
         S->w_dirty = false;
+        S->move_normalized = false;
     })
 }
 
@@ -1025,6 +1083,10 @@ int cfd_session_set_field(cfd_session* S, int f, const double* host) {
         b->upload(host, n, S->s);
         CFD_HIP(hipStreamSynchronize(S->s));
         if (f == CFD_F_W_X || f == CFD_F_W_Y || f == CFD_F_XPOS || f == CFD_F_YPOS) S->w_dirty = true;
+        if (f == CFD_F_XPOS || f == CFD_F_YPOS) S->xpos_zero = false;
+        if (f == CFD_F_X || f == CFD_F_Y || f == CFD_F_X1 || f == CFD_F_Y1 || f == CFD_F_W_X ||
+            f == CFD_F_W_Y)
+            S->move_normalized = false;
         if (f == CFD_F_DNX || f == CFD_F_DNY) {
             S->refresh_f32_geometry();
             CFD_HIP(hipStreamSynchronize(S->s));
@@ -1033,23 +1095,34 @@ int cfd_session_set_field(cfd_session* S, int f, const double* host) {
 }
 
 int cfd_session_get_scalars(cfd_session* S, double* out) {
-    CFD_TRY({ std::memcpy(out, S->sc, sizeof(S->sc)); })
+    CFD_TRY({
+        S->sync_scalars();
+        std::memcpy(out, S->sc, sizeof(S->sc));
+    })
 }
 
 int cfd_session_set_scalars(cfd_session* S, const double* in) {
-    CFD_TRY({ std::memcpy(S->sc, in, sizeof(S->sc)); })
+    CFD_TRY({
+        std::memcpy(S->sc, in, sizeof(S->sc));
+        S->put_scalars();
+    })
 }
 
-int cfd_session_timing(cfd_session* S, int enable) {
+int cfd_session_timing_mask(cfd_session* S, unsigned mask) {
     CFD_TRY({
         CFD_HIP(hipStreamSynchronize(S->s));
         S->resolve();
-        S->timing = enable != 0;
+        S->timing = mask != 0;
+        S->tmask = mask;
         for (int k = 0; k < CFD_NKTIMERS; ++k) {
             S->tacc[k] = 0.0;
             S->tcnt[k] = 0;
         }
     })
+}
+
+int cfd_session_timing(cfd_session* S, int enable) {
+    return cfd_session_timing_mask(S, enable ? (1u << CFD_NKTIMERS) - 1u : 0u);
 }
 
 int cfd_session_kernel_time(cfd_session* S, int slot, double* total_ms, long long* launches) {

@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -120,15 +121,23 @@ void launch_masas(const Tables& t, const double* area, double* M, hipStream_t s)
 // mLaplace.f90:7-58 values (structure comes from Tables)
 void launch_laplace(const Tables& t, const double* X, const double* Y, const double* dNx,
                     const double* dNy, double* lap, double* diag, hipStream_t s);
-// gcl.f90:8-46 main; mode 0 = as written, 1 = corrected y-divergence
+// gcl.f90:8-46 main; mode 0 = as written, 1 = corrected y-divergence; dt is a
+// device scalar
 void launch_gcl(const Tables& t, double* M, const double* Wx, const double* Wy,
                 const double* Wxo, const double* Wyo, const double* dNx, const double* dNy,
-                const double* area, const double* area_old, double dt, int mode, hipStream_t s);
+                const double* area, const double* area_old, const double* dt, int mode,
+                hipStream_t s);
 
 // deltat (subrutinas.f90:155-218): DT and dtmin (device), clipped at 10*dtmin
 void launch_deltat(const Tables& t, const Consts& c, const double* area, const double* T,
                    const double* VX, const double* VY, const double* WX, const double* WY,
                    double* DT, double* dtmin_out, double* partials, hipStream_t s);
+// The element pass of deltat alone: unclipped DT and one min partial per
+// block in partials; returns the number of partials (at most kRedBlocks).
+constexpr int kRedBlocks = 2048;
+int launch_deltat_partials(const Tables& t, const Consts& c, const double* area, const double* T,
+                           const double* VX, const double* VY, const double* WX, const double* WY,
+                           double* DT, double* partials, hipStream_t s);
 // Inputs of the FP32 element residual (nelem floats each), written by ESTAB
 // when nu != nullptr: nu = SHOC*CTE, tau1..3, fac = area*dtl/3.
 struct Elem32 {
@@ -137,10 +146,10 @@ struct Elem32 {
     double cte = 1.0;
 };
 
-// ESTAB (subrutinas.f90:332-446)
+// ESTAB (subrutinas.f90:332-446); dtmin is a device scalar
 void launch_estab(const Tables& t, const Consts& c, const double* U, const double* T,
                   const double* GAMM, const double* VX, const double* VY, const double* WX,
-                  const double* WY, const double* dNx, const double* dNy, double dtmin,
+                  const double* WY, const double* dNx, const double* dNy, const double* dtmin,
                   double* shoc, double* t1, double* t2, double* t3, hipStream_t s,
                   const Elem32& o32 = Elem32());
 
@@ -181,7 +190,10 @@ struct BcTable {
 };
 
 // One RK stage node update (subrutinas.f90:695-699, 707-718, 809-826) fused
-// with the gather.  write_rhs: also store the stage RHS.
+// with the gather.  final_stage: store U1, RHS and every primitive; otherwise
+// only T, the one stage output the next stage reads (every stage evaluates
+// the residual of U, subrutinas.f90:676,685, and later stages overwrite the
+// rest).
 struct StageNodes {
     const double* U;
     double *U1, *RHS, *T, *VX, *VY, *RHO, *E, *P, *RMACH;
@@ -189,11 +201,11 @@ struct StageNodes {
 };
 void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
                          const double* c_calc, const double* c_fuente, const StageNodes& nd,
-                         bool write_rhs, hipStream_t s);
+                         bool final_stage, hipStream_t s);
 // the same with FP32 slots (summed in FP64)
 void launch_stage_update32(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
                            const float* c_calc, const float* c_fuente, const StageNodes& nd,
-                           bool write_rhs, hipStream_t s);
+                           bool final_stage, hipStream_t s);
 
 // NORMALES (subrutinas.f90:7-64) on the BC table entries that are wall nodes
 void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, const int* wnode_ptr,
@@ -203,9 +215,10 @@ void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, con
 // meshMove.f90 FORCES / TRANSF / the node updates of fluidStructure
 void launch_forces(int nedge, const int* iset, const double* X, const double* Y, const double* P,
                    double xref, double yref, double* out3, hipStream_t s);
-void launch_transf(int nedge, const int* iset, const double* X, const double* Y, double ca,
-                   double sa, double yposr, double xref, double yref, double* DXPOS,
-                   double* DYPOS, hipStream_t s);
+// cs: device scalars cos(alpha), sin(alpha)
+void launch_transf(int nedge, const int* iset, const double* X, const double* Y, const double* cs,
+                   double yposr, double xref, double yref, double* DXPOS, double* DYPOS,
+                   hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // PCG (biconjGrad.f90:8-62)
@@ -227,5 +240,7 @@ PcgResult pcg_solve(PcgWork& w, int npoin, const double* A, const int* idx, cons
                     const double* diag, double* x, const double* b, const double* x_fix,
                     const int* x_fix_idx, int nfix, double tol, int maxit, int check_every,
                     hipStream_t s);
+// Copy the last solve's iteration count and final r.z to device scalars.
+void pcg_store_result(const PcgWork& w, double* iters, double* rz, hipStream_t s);
 
 }  // namespace cfd

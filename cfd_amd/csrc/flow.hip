@@ -78,10 +78,11 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
         const double* __restrict__ VEL_Y, const double* __restrict__ W_X,
         const double* __restrict__ W_Y, const double* __restrict__ dNx,
         const double* __restrict__ dNy, int nelem, double fr, double tinf, double rhoinf,
-        double dtmin, double* __restrict__ shoc, double* __restrict__ t1, double* __restrict__ t2,
-        double* __restrict__ t3, Elem32 o32) {
+        const double* __restrict__ dtminp, double* __restrict__ shoc, double* __restrict__ t1,
+        double* __restrict__ t2, double* __restrict__ t3, Elem32 o32) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nelem) return;
+    const double dtmin = *dtminp;
     int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
     double DNX[3] = {dNx[3 * e], dNx[3 * e + 1], dNx[3 * e + 2]};
     double DNY[3] = {dNy[3 * e], dNy[3 * e + 1], dNy[3 * e + 2]};
@@ -431,8 +432,10 @@ k_gather_rhs(const int* __restrict__ esup2, const double* rhs_in, const CT* __re
     if (act) st4(rhs_out + 4 * (long long)i, r0, r1, r2, r3);
 }
 
-// Gather + one RK stage for one node (subrutinas.f90:679-826, NGAS = 0)
-template <class CT, bool kFuente>
+// Gather + one RK stage for one node (subrutinas.f90:679-826, NGAS = 0).
+// kFinal = false: only T is stored (the next stage's only nodal input besides
+// U); the final stage stores everything.
+template <class CT, bool kFuente, bool kFinal>
 __global__ void __launch_bounds__(kBlock)
 k_stage(const int* __restrict__ esup2, const CT* __restrict__ c_calc,
         const CT* __restrict__ c_fuente, double rk_fact, double fr,
@@ -453,7 +456,7 @@ k_stage(const int* __restrict__ esup2, const CT* __restrict__ c_calc,
     staged_sum(c_calc, s0, s1, b, e, sbuf, r0, r1, r2, r3);
     if (kFuente) staged_sum(c_fuente, s0, s1, b, e, sbuf, r0, r1, r2, r3);
     if (!act) return;
-    if (RHS) st4(RHS + 4 * (long long)i, r0, r1, r2, r3);
+    if (kFinal && RHS) st4(RHS + 4 * (long long)i, r0, r1, r2, r3);
     double4 u = ld4(U + 4 * (long long)i);
     double f = rk_fact / M[i];
     double u0 = u.x - f * r0, u1 = u.y - f * r1, u2 = u.z - f * r2, u3 = u.w - f * r3;
@@ -465,9 +468,14 @@ k_stage(const int* __restrict__ esup2, const CT* __restrict__ c_calc,
     double vel2 = vx * vx + vy * vy;
     double pr = rho * (gamm - 1.0) * (en - 0.5 * vel2);
     double tt = pr / (rho * fr);
+    int bi = bcidx[i];
+    if (!kFinal) {  // T after the BCs: only FIX T changes it
+        if (bi >= 0 && (bcflags[bi] & 8)) tt = bcvals[8 * bi + 5];
+        T[i] = tt;
+        return;
+    }
     P[i] = pr;
     RMACH[i] = sqrt(vel2 / (tt * gamm * fr));
-    int bi = bcidx[i];
     if (bi >= 0) {
         int fl = bcflags[bi];
         const double* v = bcvals + 8 * bi;
@@ -662,20 +670,28 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, lo
 }  // namespace
 
 // ---------------------------------------------------------------------------
+int launch_deltat_partials(const Tables& t, const Consts& c, const double* area, const double* T,
+                           const double* VX, const double* VY, const double* WX, const double* WY,
+                           double* DT, double* partials, hipStream_t s) {
+    const unsigned g = std::min<unsigned>(grid_for(t.nelem), kRedBlocks);
+    k_deltat<<<g, kBlock, 0, s>>>(t.inpoel.p, area, T, VX, VY, WX, WY, DT, t.nelem, c.t_inf, c.fsafe,
+                                  partials);
+    CFD_HIP(hipGetLastError());
+    return (int)g;
+}
+
 void launch_deltat(const Tables& t, const Consts& c, const double* area, const double* T,
                    const double* VX, const double* VY, const double* WX, const double* WY,
                    double* DT, double* dtmin_out, double* partials, hipStream_t s) {
-    const unsigned g = grid_for(t.nelem);
-    k_deltat<<<g, kBlock, 0, s>>>(t.inpoel.p, area, T, VX, VY, WX, WY, DT, t.nelem, c.t_inf, c.fsafe,
-                                  partials);
-    k_min_finalize<<<1, 256, 0, s>>>(partials, (int)g, 1.0e20, __builtin_huge_val(), dtmin_out);
-    k_deltat_clip<<<g, kBlock, 0, s>>>(DT, dtmin_out, t.nelem);
+    const int g = launch_deltat_partials(t, c, area, T, VX, VY, WX, WY, DT, partials, s);
+    k_min_finalize<<<1, 256, 0, s>>>(partials, g, 1.0e20, __builtin_huge_val(), dtmin_out);
+    k_deltat_clip<<<grid_for(t.nelem), kBlock, 0, s>>>(DT, dtmin_out, t.nelem);
     CFD_HIP(hipGetLastError());
 }
 
 void launch_estab(const Tables& t, const Consts& c, const double* U, const double* T,
                   const double* GAMM, const double* VX, const double* VY, const double* WX,
-                  const double* WY, const double* dNx, const double* dNy, double dtmin,
+                  const double* WY, const double* dNx, const double* dNy, const double* dtmin,
                   double* shoc, double* t1, double* t2, double* t3, hipStream_t s,
                   const Elem32& o32) {
     k_estab<<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy,
@@ -729,35 +745,38 @@ void launch_gather_rhs(const Tables& t, const double* rhs_in, const double* c_ca
     CFD_HIP(hipGetLastError());
 }
 
+template <class CT, bool kFuente, bool kFinal>
+static void stage_launch_t(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
+                           const CT* cc, const CT* cf, const StageNodes& nd, hipStream_t s) {
+    k_stage<CT, kFuente, kFinal><<<grid_for(t.npoin), kBlock, 0, s>>>(
+        t.esup2.p, cc, cf, rk_fact, c.fr, nd.U, nd.M, nd.GAMM, nd.WX, nd.WY, bc.bcidx.p,
+        bc.flags.p, bc.vals.p, nd.U1, nd.RHS, nd.T, nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH,
+        t.npoin);
+}
+
 template <class CT>
 static void stage_launch(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
-                         const CT* cc, const CT* cf, const StageNodes& nd, bool write_rhs,
+                         const CT* cc, const CT* cf, const StageNodes& nd, bool final_stage,
                          hipStream_t s) {
-    dim3 g(grid_for(t.npoin)), b(kBlock);
-    double* rhs = write_rhs ? nd.RHS : nullptr;
-    if (cf)
-        k_stage<CT, true><<<g, b, 0, s>>>(t.esup2.p, cc, cf, rk_fact, c.fr, nd.U, nd.M, nd.GAMM, nd.WX,
-                                          nd.WY, bc.bcidx.p, bc.flags.p, bc.vals.p, nd.U1, rhs, nd.T,
-                                          nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH, t.npoin);
-    else
-        k_stage<CT, false><<<g, b, 0, s>>>(t.esup2.p, cc, cf, rk_fact, c.fr, nd.U, nd.M, nd.GAMM,
-                                           nd.WX, nd.WY, bc.bcidx.p, bc.flags.p, bc.vals.p, nd.U1, rhs,
-                                           nd.T, nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH, t.npoin);
+    if (cf && final_stage) stage_launch_t<CT, true, true>(t, c, bc, rk_fact, cc, cf, nd, s);
+    else if (cf) stage_launch_t<CT, true, false>(t, c, bc, rk_fact, cc, cf, nd, s);
+    else if (final_stage) stage_launch_t<CT, false, true>(t, c, bc, rk_fact, cc, cf, nd, s);
+    else stage_launch_t<CT, false, false>(t, c, bc, rk_fact, cc, cf, nd, s);
     CFD_HIP(hipGetLastError());
 }
 
 void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
                          const double* c_calc, const double* c_fuente, const StageNodes& nd,
-                         bool write_rhs, hipStream_t s) {
+                         bool final_stage, hipStream_t s) {
     stage_launch(t, c, bc, rk_fact, reinterpret_cast<const double4*>(c_calc),
-                 reinterpret_cast<const double4*>(c_fuente), nd, write_rhs, s);
+                 reinterpret_cast<const double4*>(c_fuente), nd, final_stage, s);
 }
 
 void launch_stage_update32(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
                            const float* c_calc, const float* c_fuente, const StageNodes& nd,
-                           bool write_rhs, hipStream_t s) {
+                           bool final_stage, hipStream_t s) {
     stage_launch(t, c, bc, rk_fact, reinterpret_cast<const float4*>(c_calc),
-                 reinterpret_cast<const float4*>(c_fuente), nd, write_rhs, s);
+                 reinterpret_cast<const float4*>(c_fuente), nd, final_stage, s);
 }
 
 void launch_element_rhs32(const Tables& t, const Consts& c, const ElemIn32& in, float* c_calc,

@@ -99,9 +99,10 @@ k_gcl(const int* __restrict__ inpoel, const int* __restrict__ esup1, const int* 
       double* __restrict__ M, const double* __restrict__ Wx, const double* __restrict__ Wy,
       const double* __restrict__ Wxo, const double* __restrict__ Wyo, const double* __restrict__ dNx,
       const double* __restrict__ dNy, const double* __restrict__ area,
-      const double* __restrict__ area_old, double dt, int mode, int npoin) {
+      const double* __restrict__ area_old, const double* __restrict__ dtp, int mode, int npoin) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npoin) return;
+    const double dt = *dtp;
     const double* wy = mode ? Wy : Wx;     // as written, the y-part reads W_x
     const double* wyo = mode ? Wyo : Wxo;
     double tot1 = 0.0, tot2 = 0.0;
@@ -152,13 +153,16 @@ __global__ void k_normales(int nw, const int* __restrict__ wnode_bc, const int* 
     flags[bi] = f;
 }
 
-// meshMove.f90:167-192, one set; per-edge terms in parallel, summed in edge order
+// meshMove.f90:167-192, one set.  The per-edge terms are formed in parallel
+// into LDS; the three running sums keep the reference's edge order, each on
+// the first lane of its own wave so the three dependency chains overlap.
 __global__ void __launch_bounds__(1024)
 k_forces(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
          const double* __restrict__ Y, const double* __restrict__ P, double xref, double yref,
          double* __restrict__ out) {
     __shared__ double dfx[1024], dfy[1024], m1[1024], m2[1024];
-    double FX = 0.0, FY = 0.0, RM = 0.0;
+    double acc = 0.0;  // wave 0: FX, wave 1: FY, wave 2: RM
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int base = 0; base < nedge; base += 1024) {
         int i = base + threadIdx.x;
         if (i < nedge) {
@@ -175,30 +179,29 @@ k_forces(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
             m2[threadIdx.x] = fx * (yc - yref);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int m = nedge - base < 1024 ? nedge - base : 1024;
-            for (int q = 0; q < m; ++q) {   // RM = RM + DFY*(XC-XR) - DFX*(YC-YR)
-                FX = FX + dfx[q];
-                FY = FY + dfy[q];
-                RM = RM + m1[q] - m2[q];
+        if (lane == 0 && wave < 3) {
+            const int m = nedge - base < 1024 ? nedge - base : 1024;
+            if (wave == 0) {
+                for (int q = 0; q < m; ++q) acc = acc + dfx[q];
+            } else if (wave == 1) {
+                for (int q = 0; q < m; ++q) acc = acc + dfy[q];
+            } else {  // RM = RM + DFY*(XC-XR) - DFX*(YC-YR)
+                for (int q = 0; q < m; ++q) acc = acc + m1[q] - m2[q];
             }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out[0] = FX;
-        out[1] = FY;
-        out[2] = RM;
-    }
+    if (lane == 0 && wave < 3) out[wave] = acc;
 }
 
 // meshMove.f90:378-389, one set
 __global__ void k_transf(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
-                         const double* __restrict__ Y, double ca, double sa, double yposr,
-                         double xref, double yref, double* __restrict__ DXPOS,
+                         const double* __restrict__ Y, const double* __restrict__ cs,
+                         double yposr, double xref, double yref, double* __restrict__ DXPOS,
                          double* __restrict__ DYPOS) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nedge) return;
+    const double ca = cs[0], sa = cs[1];
     for (int jj = 0; jj < 2; ++jj) {
         int n = iset[3 * i + jj] - 1;
         double distx = X[n] - xref;
@@ -234,7 +237,7 @@ void launch_laplace(const Tables& t, const double* X, const double* Y, const dou
 
 void launch_gcl(const Tables& t, double* M, const double* Wx, const double* Wy, const double* Wxo,
                 const double* Wyo, const double* dNx, const double* dNy, const double* area,
-                const double* area_old, double dt, int mode, hipStream_t s) {
+                const double* area_old, const double* dt, int mode, hipStream_t s) {
     k_gcl<<<grid_for(t.npoin), kBlock, 0, s>>>(t.inpoel.p, t.esup1.p, t.esup2.p, M, Wx, Wy, Wxo, Wyo,
                                                dNx, dNy, area, area_old, dt, mode, t.npoin);
     CFD_HIP(hipGetLastError());
@@ -251,15 +254,16 @@ void launch_normales(const BcTable& bc, int nw, const int* wnode_bc, const int* 
 
 void launch_forces(int nedge, const int* iset, const double* X, const double* Y, const double* P,
                    double xref, double yref, double* out3, hipStream_t s) {
+    if (nedge == 0) return;
     k_forces<<<1, 1024, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3);
     CFD_HIP(hipGetLastError());
 }
 
-void launch_transf(int nedge, const int* iset, const double* X, const double* Y, double ca,
-                   double sa, double yposr, double xref, double yref, double* DXPOS,
-                   double* DYPOS, hipStream_t s) {
+void launch_transf(int nedge, const int* iset, const double* X, const double* Y, const double* cs,
+                   double yposr, double xref, double yref, double* DXPOS, double* DYPOS,
+                   hipStream_t s) {
     if (nedge == 0) return;
-    k_transf<<<grid_for(nedge), kBlock, 0, s>>>(nedge, iset, X, Y, ca, sa, yposr, xref, yref, DXPOS,
+    k_transf<<<grid_for(nedge), kBlock, 0, s>>>(nedge, iset, X, Y, cs, yposr, xref, yref, DXPOS,
                                                 DYPOS);
     CFD_HIP(hipGetLastError());
 }

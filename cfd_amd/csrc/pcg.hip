@@ -195,7 +195,18 @@ k_fin_b(const double* __restrict__ partials, int nb, double* __restrict__ scal) 
     }
 }
 
+__global__ void k_store_result(const double* __restrict__ scal, double* __restrict__ iters,
+                               double* __restrict__ rz) {
+    *iters = scal[S_K];
+    *rz = scal[S_EOLD];
+}
+
 }  // namespace
+
+void pcg_store_result(const PcgWork& w, double* iters, double* rz, hipStream_t s) {
+    k_store_result<<<1, 1, 0, s>>>(w.scal.p, iters, rz);
+    CFD_HIP(hipGetLastError());
+}
 
 void PcgWork::ensure(int npoin) {
     if (npoin == n && r.p) return;

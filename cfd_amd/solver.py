@@ -128,8 +128,16 @@ class Session:
             self.set_scalars(**sc)
 
     # ---- timing --------------------------------------------------------------------
-    def timing(self, enable: bool = True) -> None:
-        check(lib().cfd_session_timing(self._h, int(enable)))
+    def timing(self, enable: bool = True, slots=None) -> None:
+        """HIP-event timing of every slot (enable), or only of `slots`
+        (names such as "ELEMENT"); resets the accumulators."""
+        if slots is None:
+            check(lib().cfd_session_timing(self._h, int(enable)))
+            return
+        mask = 0
+        for s in slots:
+            mask |= 1 << KTIMERS["CFD_K_" + s]
+        check(lib().cfd_session_timing_mask(self._h, C.c_uint(mask if enable else 0)))
 
     def kernel_time(self, slot: str):
         t, n = C.c_double(), C.c_longlong()

@@ -198,17 +198,23 @@ int cfd_session_destroy(cfd_session *s);
 /* RESTART free-stream branch (ns2DComp.ALE.f90:404-420) and the start-up
  * geometry (NORMALES, DERIV, MASAS, laplace: :88-100) */
 int cfd_session_init_freestream(cfd_session *s);
-/* nsteps time steps; returns after the work is queued (cfd_session_sync waits) */
+/* nsteps time steps; returns after the work is queued (cfd_session_sync
+ * waits).  The step's scalars (DTMIN, TIME, forces, ...) stay on the device;
+ * a fixed-mesh step makes no host round trip.  A moving mesh waits for each
+ * mesh solve's convergence test. */
 int cfd_session_step(cfd_session *s, int nsteps);
 int cfd_session_sync(cfd_session *s);
 /* field size: npoin, 4*npoin, nelem, 3*nelem or nnz doubles */
 int cfd_session_field_size(cfd_session *s, int field, long long *count);
 int cfd_session_get_field(cfd_session *s, int field, double *host);
 int cfd_session_set_field(cfd_session *s, int field, const double *host);
+/* the CFD_S_* scalars; get waits for queued steps */
 int cfd_session_get_scalars(cfd_session *s, double *scalars /* CFD_NSCALARS */);
 int cfd_session_set_scalars(cfd_session *s, const double *scalars);
-/* per-slot kernel time: enable resets the accumulators */
+/* per-slot kernel time: enable (all slots) resets the accumulators */
 int cfd_session_timing(cfd_session *s, int enable);
+/* the same for the slots whose bit (1 << CFD_K_*) is set in mask */
+int cfd_session_timing_mask(cfd_session *s, unsigned mask);
 int cfd_session_kernel_time(cfd_session *s, int slot, double *total_ms, long long *launches);
 
 #ifdef __cplusplus
