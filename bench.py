@@ -144,7 +144,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": read_traffic(f"element_{args.precision}_traffic_cfg2.txt"),
+            "traffic": (read_traffic(f"k_element_rhs_{args.precision}_cylinder1024x2048_traffic.txt")
+                        if (args.nr, args.nt) == (1024, 2048) else None),
             "algorithmic_bytes_per_launch": bytes_elem,
             "avg_launch_us": avg_elem_s * 1e6,
         },

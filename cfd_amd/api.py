@@ -211,3 +211,20 @@ def biCG(spMtx, spIdx, spRowptr, diagMtx, x, b, x_fix, x_fixIdx):
                          dptr(_f64(diagMtx)), dptr(x), dptr(_f64(b)), dptr(xf) if xf.size else None,
                          iptr(fi) if fi.size else None, int(fi.size), C.byref(it), C.byref(rz)))
     return it.value, rz.value
+
+
+def biCG_ex(spMtx, spIdx, spRowptr, diagMtx, x, b, x_fix, x_fixIdx, tol=1e-10, maxit=1000, rel=0):
+    """biCG with the stopping rule as arguments (rel=1: tests relative to the
+    residual of the guess that is zero off the fixed nodes) -> (iters, |r.z|,
+    converged)"""
+    assert x.dtype == np.float64 and x.flags.c_contiguous
+    n = x.size
+    it, rz, cv = C.c_int(), C.c_double(), C.c_int()
+    xf = _f64(x_fix)
+    fi = _i32(x_fixIdx)
+    check(lib().cfd_bicg_ex(n, dptr(_f64(spMtx)), iptr(_i32(spIdx)), iptr(_i32(spRowptr)),
+                            dptr(_f64(diagMtx)), dptr(x), dptr(_f64(b)),
+                            dptr(xf) if xf.size else None, iptr(fi) if fi.size else None,
+                            int(fi.size), C.c_double(tol), int(maxit), int(rel), C.byref(it),
+                            C.byref(rz), C.byref(cv)))
+    return it.value, rz.value, bool(cv.value)

@@ -67,20 +67,42 @@ def build_library(force: bool = True, jobs: int = 8) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(PKG, "fortran")
+ABI_DEMO = os.path.join(OBJ, "abi_demo")
+
+
+def build_fortran() -> str | None:
+    """The Fortran interface module (cfd_amd/fortran/cfd_abi.f90) and the demo
+    driver that calls the library through it, linked against libcfd_amd.so
+    (found next to the package at run time).  None when flang is absent."""
+    fc = shutil.which("flang")
+    if fc is None:
+        return None
+    os.makedirs(OBJ, exist_ok=True)
+    abi_o = os.path.join(OBJ, "cfd_abi_f.o")
+    demo_o = os.path.join(OBJ, "abi_demo_f.o")
+    _run([fc, "-O2", "-J", OBJ, "-c", os.path.join(FORTRAN, "cfd_abi.f90"), "-o", abi_o])
+    _run([fc, "-O2", "-J", OBJ, "-c", os.path.join(FORTRAN, "abi_demo.f90"), "-o", demo_o])
+    _run([fc, demo_o, abi_o, "-L" + PKG, "-lcfd_amd", "-Wl,-rpath,$ORIGIN/..", "-o", ABI_DEMO])
+    return ABI_DEMO
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile"])
 
 
 def build_reference() -> bool:
-    """Compiles the reference into oracle/_ref when its sources are present."""
+    """Compiles the reference into oracle/_ref when its sources are present,
+    and the Fortran shims against the reference's own module sources."""
     if not os.path.isdir("/root/reference") or shutil.which("flang") is None:
         return False
-    _run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile.ref"])
+    _run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile.ref", "all", "shims"])
     return True
 
 
 if __name__ == "__main__":
     build_library(force="--incremental" not in sys.argv)
+    build_fortran()
     if "--all" in sys.argv:
         build_oracle()
         build_reference()

@@ -584,8 +584,17 @@ int cfd_rk(cfd_mesh* m, const cfd_consts* c, const cfd_bc_lists* bc, int nrk, do
 int cfd_bicg(int npoin, const double* spMtx, const int* spIdx, const int* spRowptr,
              const double* diagMtx, double* x, const double* b, const double* x_fix,
              const int* x_fixIdx, int nfix, int* iters, double* rz) {
+    return cfd_bicg_ex(npoin, spMtx, spIdx, spRowptr, diagMtx, x, b, x_fix, x_fixIdx, nfix, 1.0e-10,
+                       1000, 0, iters, rz, nullptr);
+}
+
+int cfd_bicg_ex(int npoin, const double* spMtx, const int* spIdx, const int* spRowptr,
+                const double* diagMtx, double* x, const double* b, const double* x_fix,
+                const int* x_fixIdx, int nfix, double tol, int maxit, int rel, int* iters,
+                double* rz, int* converged) {
     CFD_TRY({
         CFD_ARG(npoin > 0 && spMtx && spIdx && spRowptr && diagMtx && x && b, "null argument");
+        CFD_ARG(tol > 0.0 && maxit >= 0 && (rel == 0 || rel == 1), "tol > 0, maxit >= 0, rel 0/1");
         CFD_ARG(nfix >= 0 && (nfix == 0 || (x_fix && x_fixIdx)), "fixed-node lists");
         for (int q = 0; q < nfix; ++q)
             CFD_ARG(x_fixIdx[q] >= 1 && x_fixIdx[q] <= npoin, "x_fixIdx out of range");
@@ -607,12 +616,13 @@ int cfd_bicg(int npoin, const double* spMtx, const int* spIdx, const int* spRowp
         XF.upload(x_fix, nfix, s);
         FI.upload(x_fixIdx, nfix, s);
         PcgWork w;
-        PcgResult res = pcg_solve(w, npoin, A.p, I.p, R.p, D.p, X.p, B.p, XF.p, FI.p, nfix, 1.0e-10,
-                                  1000, 8, s);
+        pcg_launch(w, npoin, A.p, I.p, R.p, D.p, X.p, B.p, XF.p, FI.p, nfix, tol, maxit, rel, s);
+        PcgResult res = pcg_result(w, s);
         X.download(x, npoin, s);
         CFD_HIP(hipStreamSynchronize(s));
         if (iters) *iters = res.iters;
         if (rz) *rz = res.rz;
+        if (converged) *converged = res.converged;
     })
 }
 
@@ -689,12 +699,12 @@ struct cfd_session {
     int ucur = 0;
     DBuf<double> RHS, T, VX, VY, WX, WY, X, Y, X1, Y1, RHO, E, P, RMACH, M, GAMM, XPOS, YPOS;
     DBuf<double> DTL, DT, SHOC, T1, T2, T3, AREA, DNX, DNY, HH, HHX, HHY, AREA_OLD, WXO, WYO;
-    DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, BZERO, partials;
+    DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, POSAUY, BZERO, partials;
     // FP32 element path: binary32 shape gradients, ESTAB outputs, slots
     DBuf<float> DNX32, DNY32, NU32, TA32, TB32, TC32, FAC32, c32, f32b;
     DBuf<int> ILAUX, ISET;
     BcDevice bc;
-    PcgWork pcg;
+    PcgWork pcg, pcg_y;  // the x and y mesh solves
     DBuf<double> ctl;                  // session scalars on the device (X_NCTL slots)
     double sc[CFD_NSCALARS] = {0};     // host copy, refreshed by sync_scalars()
     bool w_dirty = false;
@@ -781,28 +791,33 @@ struct cfd_session {
         launch_to_f32(DNY.p, DNY32.p, 3LL * ne, s);
     }
 
-    // One biCG mesh solve of fluidStructure (meshMove.f90:91-104), then
-    // X += XPOS, X1 += XPOS, W = XPOS/DTMIN.  The host waits for the solve's
-    // convergence test.
-    void mesh_solve(DBuf<double>& dpos, DBuf<double>& pos, DBuf<double>& coord, DBuf<double>& coord1,
-                    DBuf<double>& w, int which) {
+    // The two biCG mesh solves of fluidStructure (meshMove.f90:91-126), then
+    // X += XPOS, X1 += XPOS, W_X = XPOS/DTMIN and the same for y.  The y
+    // solve reads nothing the x update writes (same matrix, DYPOS from
+    // TRANSF), so the two run interleaved.
+    void mesh_solves() {
         const int nn = d.nmove + d.nfix_move;
-        if (nn > 0)
-            k_pos_aux<<<grid_for(nn), kBlock, 0, s>>>(ILAUX.p, dpos.p, POSAUX.p, d.nmove, nn);
+        if (nn > 0) {
+            k_pos_aux<<<grid_for(nn), kBlock, 0, s>>>(ILAUX.p, DXPOS.p, POSAUX.p, d.nmove, nn);
+            k_pos_aux<<<grid_for(nn), kBlock, 0, s>>>(ILAUX.p, DYPOS.p, POSAUY.p, d.nmove, nn);
+        }
         tb(CFD_K_PCG);
-        double tol = d.pcg_tol > 0.0 ? d.pcg_tol : 1.0e-10;
-        int maxit = d.pcg_maxit > 0 ? d.pcg_maxit : 1000;
-        pcg_solve(pcg, np, LAP.p, m->t.idx.p, m->t.rowptr.p, LDIAG.p, pos.p, BZERO.p, POSAUX.p,
-                  ILAUX.p, nn, tol, maxit, 8, s);
+        const double tol = d.pcg_tol > 0.0 ? d.pcg_tol : 1.0e-10;
+        const int maxit = d.pcg_maxit > 0 ? d.pcg_maxit : 1000;
+        const PcgSystem sys[2] = {{&pcg, XPOS.p, BZERO.p, POSAUX.p, ILAUX.p, nn},
+                                  {&pcg_y, YPOS.p, BZERO.p, POSAUY.p, ILAUX.p, nn}};
+        pcg_launch_multi(sys, 2, np, LAP.p, m->t.idx.p, m->t.rowptr.p, LDIAG.p, tol, maxit,
+                         d.pcg_rel, s);
         te(CFD_K_PCG);
-        pcg_store_result(pcg, scal(which ? CFD_S_PCG_ITY : CFD_S_PCG_ITX),
-                         scal(which ? CFD_S_PCG_RZY : CFD_S_PCG_RZX), s);
-        k_move<<<grid_for(np), kBlock, 0, s>>>(coord.p, coord1.p, w.p, pos.p, scal(CFD_S_DTMIN), np);
+        pcg_store_result(pcg, scal(CFD_S_PCG_ITX), scal(CFD_S_PCG_RZX), s);
+        pcg_store_result(pcg_y, scal(CFD_S_PCG_ITY), scal(CFD_S_PCG_RZY), s);
+        k_move<<<grid_for(np), kBlock, 0, s>>>(X.p, X1.p, WX.p, XPOS.p, scal(CFD_S_DTMIN), np);
+        k_move<<<grid_for(np), kBlock, 0, s>>>(Y.p, Y1.p, WY.p, YPOS.p, scal(CFD_S_DTMIN), np);
         CFD_HIP(hipGetLastError());
     }
 
     // One time step, queued on the stream without waiting for the device
-    // (unless the mesh moves: the mesh solves test convergence on the host).
+    // (except inside the mesh solves: the host polls PCG's done flag).
     void step() {
         const Tables& t = m->t;
         tb(CFD_K_STEP);
@@ -876,8 +891,7 @@ struct cfd_session {
             if (d.nmove > 0)
                 launch_transf(d.nsetedges, ISET.p, X.p, Y.p, scal(X_CA), 0.0, d.xref, d.yref,
                               DXPOS.p, DYPOS.p, s);
-            mesh_solve(DXPOS, XPOS, X, X1, WX, 0);
-            mesh_solve(DYPOS, YPOS, Y, Y1, WY, 1);
+            mesh_solves();
             move_normalized = true;
         }
 
@@ -996,6 +1010,7 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
             S->c_fuente.alloc(12 * (size_t)ne);
         }
         S->POSAUX.alloc(std::max(1, nn));
+        S->POSAUY.alloc(std::max(1, nn));
         S->ILAUX.alloc(std::max(1, nn));
         S->ILAUX.upload(S->ilaux_h.data(), nn, s);
         S->ISET.alloc(std::max<size_t>(3, S->iset_h.size()));
@@ -1016,6 +1031,7 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         BcHost h = build_bc_host(np, &d->bc);
         S->bc.upload(h, np, &d->bc, s);
         S->pcg.ensure(np);
+        S->pcg_y.ensure(np);
         S->sc[CFD_S_BANDERA] = 1.0;
         S->put_scalars();
         *out = S.release();

@@ -82,6 +82,23 @@ struct DBuf {
     }
 };
 
+// Owned pinned host allocation (device-to-host reads without staging).
+template <class T>
+struct HostPinned {
+    T* p = nullptr;
+    HostPinned() = default;
+    HostPinned(const HostPinned&) = delete;
+    HostPinned& operator=(const HostPinned&) = delete;
+    ~HostPinned() {
+        if (p) (void)hipHostFree(p);
+    }
+    void alloc(size_t count) {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        CFD_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count));
+    }
+};
+
 constexpr int kBlock = 256;
 inline unsigned grid_for(size_t n, int block = kBlock) {
     size_t g = (n + block - 1) / block;
@@ -228,18 +245,39 @@ struct PcgWork {
     DBuf<double> r, z, y, pa, pb, partial;
     DBuf<double> scal;         // see pcg.hip for the slot meaning
     DBuf<int> fixflag;         // n: 1 on Dirichlet nodes
+    HostPinned<double> hpin;   // host copy of scal for the polls
     void ensure(int npoin);
 };
+// One system of a multi-system solve: its own work space, x, b and fixed
+// values; the matrix is shared.
+struct PcgSystem {
+    PcgWork* w;
+    double* x;
+    const double* b;
+    const double* x_fix;
+    const int* x_fix_idx;
+    int nfix;
+};
+// Several independent solves on one matrix, interleaved on stream s with
+// shared polls.
+void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A, const int* idx,
+                      const int* rowptr, const double* diag, double tol, int maxit, int rel,
+                      hipStream_t s);
 struct PcgResult {
     int iters;
     double rz;
     int converged;
 };
+// One solve on stream s; the host polls the done flag between batches of
+// iterations, so the call returns once the loop has ended on the device.
 // x (device) in/out; b (device); x_fix/x_fix_idx device arrays of nfix.
-PcgResult pcg_solve(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
-                    const double* diag, double* x, const double* b, const double* x_fix,
-                    const int* x_fix_idx, int nfix, double tol, int maxit, int check_every,
-                    hipStream_t s);
+// rel = 0: the reference's absolute tests against tol; rel = 1: relative
+// (see pcg.hip).
+void pcg_launch(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
+                const double* diag, double* x, const double* b, const double* x_fix,
+                const int* x_fix_idx, int nfix, double tol, int maxit, int rel, hipStream_t s);
+// Waits for the stream and returns the last solve's outcome.
+PcgResult pcg_result(const PcgWork& w, hipStream_t s);
 // Copy the last solve's iteration count and final r.z to device scalars.
 void pcg_store_result(const PcgWork& w, double* iters, double* rz, hipStream_t s);
 

@@ -10,15 +10,23 @@
 //
 // On the device one iteration is two full-grid kernels, each followed by a
 // one-block finalize that combines the block partials and updates the
-// scalars in device memory:
+// scalars:
 //   A: x += alfa p (the previous iteration's update), r -= alfa y, z = r/diag,
 //      partials of r.z;                       finalize: e, beta, k.
 //   B: each row forms p_new = beta p_old + z for its columns on the fly, y = A
 //      p_new, stores its own p_new (double-buffered p), partials of p.y;
 //                                             finalize: alfa, the loop test.
-// The host polls a done flag every few iterations.  Dot products combine
-// block partials in a fixed tree, so the iteration sequence is deterministic
-// (not the serial summation order of the reference).
+// The scalars live in device memory.  Kernels queued after a loop has ended
+// return at once (done flag), so the host queues batches of iterations
+// between polls.  Several independent systems on one matrix (the x and y mesh
+// solves) run interleaved and share the polls.  Dot products combine block
+// partials in a fixed tree, so the iteration sequence is deterministic (not
+// the serial summation order of the reference).
+//
+// Relative mode (rel = 1): the two tests use tol^2 times the r.r and r.z of
+// the residual of the guess that is zero off the fixed nodes, so the stopping
+// point does not depend on the initial guess (warm starts compare at equal
+// tolerance).
 #include "cfd_internal.hpp"
 #include "reduce.hpp"
 
@@ -26,11 +34,21 @@ namespace cfd {
 namespace {
 
 // scal[] slots
-enum { S_ALFA = 0, S_BETA, S_EOLD, S_ENEW, S_PY, S_RR, S_K, S_DONE, S_STOP, S_CONV, S_TOL, S_MAXIT };
+enum {
+    S_ALFA = 0, S_BETA, S_EOLD, S_ENEW, S_PY, S_RR, S_K, S_DONE, S_STOP, S_CONV, S_TOL, S_MAXIT,
+    S_TOLR, S_TOLZ, S_NSCAL = 16
+};
 
-__global__ void k_fix_init(int n, int* __restrict__ fixflag) {
+// fixflag = 0; scalars reset (block 0)
+__global__ void k_fix_init(int n, int* __restrict__ fixflag, double* __restrict__ scal, double tol,
+                           int maxit) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) fixflag[i] = 0;
+    if (i == 0) {
+        for (int q = 0; q < S_NSCAL; ++q) scal[q] = 0.0;
+        scal[S_TOL] = tol;
+        scal[S_MAXIT] = (double)maxit;
+    }
 }
 
 __global__ void k_fix_set(int nfix, const int* __restrict__ idx, const double* __restrict__ x_fix,
@@ -42,41 +60,62 @@ __global__ void k_fix_set(int nfix, const int* __restrict__ idx, const double* _
     x[i] = 1.0 * x_fix[q];
 }
 
-__global__ void k_scal_init(double* __restrict__ scal, double tol, int maxit) {
-    for (int q = 0; q < 16; ++q) scal[q] = 0.0;
-    scal[S_TOL] = tol;
-    scal[S_MAXIT] = (double)maxit;
-}
-
-// r = -1*(A x) + b with fixed rows y = 1e30 x and r(fix) = 0; partials of r.r
+// r = -1*(A x) + b with fixed rows y = 1e30 x and r(fix) = 0; r.r and, in
+// relative mode, rd.rd and rd.(rd/diag) for the residual rd of the guess that
+// is zero off the fixed nodes; the first residual test
 __global__ void __launch_bounds__(kBlock)
 k_residual(int n, const double* __restrict__ A, const int* __restrict__ idx,
            const int* __restrict__ rowptr, const int* __restrict__ fixflag,
-           const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
+           const double* __restrict__ diag, const double* __restrict__ x,
+           const double* __restrict__ b, double* __restrict__ r, int rel,
            double* __restrict__ partials) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v = 0.0;
+    double v = 0.0, vdr = 0.0, vdz = 0.0;
     if (i < n) {
-        double ri;
-        if (fixflag[i]) {
-            ri = 0.0;
-        } else {
-            double dot = 0.0;
-            for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) dot = dot + A[j] * x[idx[j] - 1];
+        double ri = 0.0;
+        if (!fixflag[i]) {
+            double dot = 0.0, dotd = 0.0;
+            for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) {
+                const int c = idx[j] - 1;
+                const double xc = x[c];
+                dot = dot + A[j] * xc;
+                if (rel && fixflag[c]) dotd = dotd + A[j] * xc;
+            }
             ri = -1.0 * dot + b[i];
+            if (rel) {
+                const double rd = -1.0 * dotd + b[i];
+                vdr = rd * rd;
+                vdz = rd * (rd / diag[i]);
+            }
         }
         r[i] = ri;
         v = ri * ri;
     }
+    const int nb = gridDim.x;
     block_partial(v, OpSum(), partials);
+    if (rel) {
+        block_partial(vdr, OpSum(), partials + nb);
+        block_partial(vdz, OpSum(), partials + 2 * nb);
+    }
 }
 
+// the first residual test (one block)
 __global__ void __launch_bounds__(256)
-k_fin_residual(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
-    double rr = combine_partials(partials, nb, 0.0, OpSum());
+k_fin_residual(const double* __restrict__ partials, int nb, int rel, double* __restrict__ scal) {
+    const double rr = combine_partials(partials, nb, 0.0, OpSum());
+    double dr = 0.0, dz = 0.0;
+    if (rel) {
+        dr = combine_partials(partials + nb, nb, 0.0, OpSum());
+        dz = combine_partials(partials + 2 * nb, nb, 0.0, OpSum());
+    }
     if (threadIdx.x == 0) {
+        const double tol = scal[S_TOL];
+        const double tolr = rel ? tol * tol * dr : tol;
+        const double tolz = rel ? tol * tol * dz : tol;
+        scal[S_TOLR] = tolr;
+        scal[S_TOLZ] = tolz;
         scal[S_RR] = rr;
-        if (rr < scal[S_TOL]) {
+        if (rr < tolr) {
             scal[S_DONE] = 1.0;
             scal[S_CONV] = 1.0;
             scal[S_EOLD] = rr;
@@ -87,7 +126,8 @@ k_fin_residual(const double* __restrict__ partials, int nb, double* __restrict__
 // first preconditioned residual: z = r/diag, partials of r.z
 __global__ void __launch_bounds__(kBlock)
 k_first_z(int n, const double* __restrict__ r, const double* __restrict__ diag,
-          double* __restrict__ z, const double* __restrict__ scal, double* __restrict__ partials) {
+          double* __restrict__ z, const double* __restrict__ scal,
+          double* __restrict__ partials) {
     if (scal[S_DONE] != 0.0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
@@ -102,7 +142,7 @@ k_first_z(int n, const double* __restrict__ r, const double* __restrict__ diag,
 __global__ void __launch_bounds__(256)
 k_fin_first_z(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
     if (scal[S_DONE] != 0.0) return;
-    double e = combine_partials(partials, nb, 0.0, OpSum());
+    const double e = combine_partials(partials, nb, 0.0, OpSum());
     if (threadIdx.x == 0) scal[S_ENEW] = e;
 }
 
@@ -137,11 +177,11 @@ k_fin_a(const double* __restrict__ partials, int nb, double* __restrict__ scal) 
     if (scal[S_STOP] != 0.0) {
         if (threadIdx.x == 0) {
             scal[S_DONE] = 1.0;
-            scal[S_CONV] = fabs(scal[S_EOLD]) > scal[S_TOL] ? 0.0 : 1.0;
+            scal[S_CONV] = fabs(scal[S_EOLD]) > scal[S_TOLZ] ? 0.0 : 1.0;
         }
         return;
     }
-    double e = combine_partials(partials, nb, 0.0, OpSum());
+    const double e = combine_partials(partials, nb, 0.0, OpSum());
     if (threadIdx.x == 0) {
         scal[S_K] = scal[S_K] + 1.0;
         scal[S_ENEW] = e;
@@ -184,13 +224,13 @@ k_iter_b(int n, const double* __restrict__ A, const int* __restrict__ idx,
 __global__ void __launch_bounds__(256)
 k_fin_b(const double* __restrict__ partials, int nb, double* __restrict__ scal) {
     if (scal[S_DONE] != 0.0 || scal[S_STOP] != 0.0) return;
-    double py = combine_partials(partials, nb, 0.0, OpSum());
+    const double py = combine_partials(partials, nb, 0.0, OpSum());
     if (threadIdx.x == 0) {
         double e = scal[S_ENEW];
         scal[S_PY] = py;
         scal[S_ALFA] = e / py;
         scal[S_EOLD] = e;
-        bool more = fabs(e) > scal[S_TOL] && scal[S_K] < scal[S_MAXIT];
+        bool more = fabs(e) > scal[S_TOLZ] && scal[S_K] < scal[S_MAXIT];
         if (!more) scal[S_STOP] = 1.0;
     }
 }
@@ -218,62 +258,95 @@ void PcgWork::ensure(int npoin) {
     pa.alloc(npoin);
     pb.alloc(npoin);
     fixflag.alloc(npoin);
-    partial.alloc(nblocks);
-    scal.alloc(16);
+    partial.alloc(3 * (size_t)nblocks);
+    scal.alloc(S_NSCAL);
+    hpin.alloc(S_NSCAL);
 }
 
-PcgResult pcg_solve(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
-                    const double* diag, double* x, const double* b, const double* x_fix,
-                    const int* x_fix_idx, int nfix, double tol, int maxit, int check_every,
-                    hipStream_t s) {
-    w.ensure(npoin);
-    const int nb = w.nblocks;
-    const dim3 g(nb), blk(kBlock), one(1), fin(256);
-    double* scal = w.scal.p;
-    double* part = w.partial.p;
-    double h[16];
-    auto poll = [&]() {
-        CFD_HIP(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, s));
-        CFD_HIP(hipStreamSynchronize(s));
+void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A, const int* idx,
+                      const int* rowptr, const double* diag, double tol, int maxit, int rel,
+                      hipStream_t s) {
+    CFD_CHECK(npoin > 0, "arg: PCG needs npoin > 0");
+    const dim3 g(grid_for(npoin)), blk(kBlock);
+    std::vector<int> cur(nsys, 0);  // pbuf[cur] holds the current p
+    const int nb = (int)g.x;
+    const dim3 one(1), fin(256);
+    auto iter_a = [&](int k) {
+        PcgWork& w = *sys[k].w;
+        double* pbuf[2] = {w.pa.p, w.pb.p};
+        k_iter_a<<<g, blk, 0, s>>>(npoin, sys[k].x, pbuf[cur[k]], w.r.p, w.y.p, diag, w.z.p,
+                                   w.scal.p, w.partial.p);
+        k_fin_a<<<one, fin, 0, s>>>(w.partial.p, nb, w.scal.p);
     };
-    double* pbuf[2] = {w.pa.p, w.pb.p};
-    auto iter_a = [&](int cur) {
-        k_iter_a<<<g, blk, 0, s>>>(npoin, x, pbuf[cur], w.r.p, w.y.p, diag, w.z.p, scal, part);
-        k_fin_a<<<one, fin, 0, s>>>(part, nb, scal);
+    auto iter_b = [&](int k, int first) {
+        PcgWork& w = *sys[k].w;
+        double* pbuf[2] = {w.pa.p, w.pb.p};
+        k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[cur[k]], w.z.p,
+                                   pbuf[cur[k] ^ 1], w.y.p, first, w.scal.p, w.partial.p);
+        k_fin_b<<<one, fin, 0, s>>>(w.partial.p, nb, w.scal.p);
     };
-    auto iter_b = [&](int cur, int first) {
-        k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[cur], w.z.p,
-                                   pbuf[cur ^ 1], w.y.p, first, scal, part);
-        k_fin_b<<<one, fin, 0, s>>>(part, nb, scal);
-    };
-    k_scal_init<<<one, one, 0, s>>>(scal, tol, maxit);
-    k_fix_init<<<g, blk, 0, s>>>(npoin, w.fixflag.p);
-    if (nfix > 0)
-        k_fix_set<<<grid_for(nfix), blk, 0, s>>>(nfix, x_fix_idx, x_fix, w.fixflag.p, x);
-    k_residual<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, x, b, w.r.p, part);
-    k_fin_residual<<<one, fin, 0, s>>>(part, nb, scal);
-    k_first_z<<<g, blk, 0, s>>>(npoin, w.r.p, diag, w.z.p, scal, part);
-    k_fin_first_z<<<one, fin, 0, s>>>(part, nb, scal);
-    iter_b(1, 1);  // p0 = z0 into pbuf[0]
+    for (int k = 0; k < nsys; ++k) {
+        PcgWork& w = *sys[k].w;
+        w.ensure(npoin);
+        k_fix_init<<<g, blk, 0, s>>>(npoin, w.fixflag.p, w.scal.p, tol, maxit);
+        if (sys[k].nfix > 0)
+            k_fix_set<<<grid_for(sys[k].nfix), blk, 0, s>>>(sys[k].nfix, sys[k].x_fix_idx,
+                                                            sys[k].x_fix, w.fixflag.p, sys[k].x);
+        k_residual<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, diag, sys[k].x, sys[k].b,
+                                     w.r.p, rel, w.partial.p);
+        k_fin_residual<<<one, fin, 0, s>>>(w.partial.p, nb, rel, w.scal.p);
+        k_first_z<<<g, blk, 0, s>>>(npoin, w.r.p, diag, w.z.p, w.scal.p, w.partial.p);
+        k_fin_first_z<<<one, fin, 0, s>>>(w.partial.p, nb, w.scal.p);
+        cur[k] = 1;
+        iter_b(k, 1);  // p0 = z0 into pbuf[0]
+        cur[k] = 0;
+    }
     CFD_HIP(hipGetLastError());
-    poll();        // most warm-started mesh solves stop here (r.r < tol)
-    int cur = 0;   // pbuf[cur] holds the current p
-    const int chunk = check_every > 0 ? check_every : 8;
-    for (int launched = 0; h[S_DONE] == 0.0; launched += chunk) {
-        if (h[S_STOP] != 0.0) {  // one A applies the final x update and ends the solve
-            iter_a(cur);
-            poll();
-            break;
+    std::vector<char> done(nsys, 0), stop(nsys, 0);
+    auto poll = [&]() {
+        for (int k = 0; k < nsys; ++k)
+            CFD_HIP(hipMemcpyAsync(sys[k].w->hpin.p, sys[k].w->scal.p, S_NSCAL * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+        CFD_HIP(hipStreamSynchronize(s));
+        bool all = true;
+        for (int k = 0; k < nsys; ++k) {
+            done[k] = sys[k].w->hpin.p[S_DONE] != 0.0;
+            stop[k] = sys[k].w->hpin.p[S_STOP] != 0.0;
+            all = all && done[k];
         }
+        return all;
+    };
+    // most warm-started mesh solves stop at the first residual test; batches
+    // grow while a solve runs on (few polls, few no-op launches)
+    int chunk = 4;
+    for (long launched = 0; !poll(); launched += chunk) {
+        for (int k = 0; k < nsys; ++k)
+            if (!done[k] && stop[k]) iter_a(k);  // the final x update ends the solve
         for (int q = 0; q < chunk; ++q) {
-            iter_a(cur);
-            iter_b(cur, 0);
-            cur ^= 1;
+            for (int k = 0; k < nsys; ++k) {
+                if (done[k] || stop[k]) continue;
+                iter_a(k);
+                iter_b(k, 0);
+                cur[k] ^= 1;
+            }
         }
         CFD_HIP(hipGetLastError());
-        poll();
-        CFD_CHECK(launched <= 4 * maxit + 64, "PCG did not terminate");
+        CFD_CHECK(launched <= 4L * maxit + 64, "PCG did not terminate");
+        chunk = std::min(chunk * 2, 16);
     }
+}
+
+void pcg_launch(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
+                const double* diag, double* x, const double* b, const double* x_fix,
+                const int* x_fix_idx, int nfix, double tol, int maxit, int rel, hipStream_t s) {
+    PcgSystem sys{&w, x, b, x_fix, x_fix_idx, nfix};
+    pcg_launch_multi(&sys, 1, npoin, A, idx, rowptr, diag, tol, maxit, rel, s);
+}
+
+PcgResult pcg_result(const PcgWork& w, hipStream_t s) {
+    double h[S_NSCAL];
+    CFD_HIP(hipMemcpyAsync(h, w.scal.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    CFD_HIP(hipStreamSynchronize(s));
     PcgResult res;
     res.iters = (int)h[S_K];
     res.rz = h[S_EOLD];

@@ -1,0 +1,223 @@
+! Fortran 2003 interface to libcfd_amd.so: the C ABI of include/cfd_abi.h
+! (host-pointer entries) as iso_c_binding interfaces, plus cfd_check, which
+! turns a non-zero return code into ERROR STOP with the library's message.
+! Arrays follow the reference's conventions: 1-based ids, 0-based offsets,
+! column-major U(4,npoin), inpoel(3,nelem), dNx(3,nelem).
+module cfd_abi
+    use iso_c_binding
+    implicit none
+
+    ! InputData constants the path reads (dataLoader.f90:5-8, derived :59-64)
+    type, bind(c) :: cfd_consts
+        real(c_double) :: gama, fr, fmu, fk, fcv
+        real(c_double) :: t_inf, rho_inf, u_inf, v_inf
+        real(c_double) :: cte, fsafe
+    end type cfd_consts
+
+    ! boundary lists as left by dataLoader (dataLoader.f90:130-268)
+    type, bind(c) :: cfd_bc_lists
+        integer(c_int) :: nfixv
+        type(c_ptr) :: ifixv_node, rfixv_valuex, rfixv_valuey
+        integer(c_int) :: nfixrho
+        type(c_ptr) :: ifixrho_node, rfixrho_value
+        integer(c_int) :: nfixt
+        type(c_ptr) :: ifixt_node, rfixt_value
+        integer(c_int) :: nwall
+        type(c_ptr) :: wall
+    end type cfd_bc_lists
+
+    ! RK's module arrays (subrutinas.f90:645-849), nodal and per element
+    type, bind(c) :: cfd_rk_nodes
+        type(c_ptr) :: u1, rhs, t, vel_x, vel_y, rho, e, p, rmach
+        type(c_ptr) :: u, w_x, w_y, m, gamm
+    end type cfd_rk_nodes
+    type, bind(c) :: cfd_rk_elems
+        type(c_ptr) :: dnx, dny, area
+        type(c_ptr) :: shoc, t_sugn1, t_sugn2, t_sugn3
+    end type cfd_rk_elems
+
+    interface
+        integer(c_int) function cfd_abi_version() bind(c, name='cfd_abi_version')
+            import :: c_int
+        end function
+        type(c_ptr) function cfd_last_error() bind(c, name='cfd_last_error')
+            import :: c_ptr
+        end function
+        integer(c_int) function cfd_device_count(count) bind(c, name='cfd_device_count')
+            import :: c_int
+            integer(c_int), intent(out) :: count
+        end function
+
+        ! mesh context (PointNeighbor / Mlaplace tables on the device)
+        integer(c_int) function cfd_mesh_create(m, nelem, npoin, inpoel) &
+                bind(c, name='cfd_mesh_create')
+            import :: c_int, c_ptr
+            type(c_ptr), intent(out) :: m
+            integer(c_int), value :: nelem, npoin
+            integer(c_int), intent(in) :: inpoel(*)
+        end function
+        integer(c_int) function cfd_mesh_destroy(m) bind(c, name='cfd_mesh_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: m
+        end function
+        integer(c_int) function cfd_mesh_sizes(m, npsup, nnz) bind(c, name='cfd_mesh_sizes')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: m
+            integer(c_int), intent(out) :: npsup, nnz
+        end function
+        ! PointNeighbor::getEsup (pointNeighbor.f90:5-43)
+        integer(c_int) function cfd_get_esup(m, esup1, esup2) bind(c, name='cfd_get_esup')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: m
+            integer(c_int), intent(out) :: esup1(*), esup2(*)
+        end function
+        ! PointNeighbor::getPsup (pointNeighbor.f90:45-91)
+        integer(c_int) function cfd_get_psup(m, psup1, psup2) bind(c, name='cfd_get_psup')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: m
+            integer(c_int), intent(out) :: psup1(*), psup2(*)
+        end function
+        ! Mlaplace initialize (mLaplace.f90:60-94)
+        integer(c_int) function cfd_lap_structure(m, lap_rowptr, lap_idx) &
+                bind(c, name='cfd_lap_structure')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: m
+            integer(c_int), intent(out) :: lap_rowptr(*), lap_idx(*)
+        end function
+        ! Mlaplace::laplace (mLaplace.f90:7-58)
+        integer(c_int) function cfd_laplace(m, x, y, dnx, dny, lap_sparse, lap_diag) &
+                bind(c, name='cfd_laplace')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(in) :: x(*), y(*), dnx(*), dny(*)
+            real(c_double), intent(out) :: lap_sparse(*), lap_diag(*)
+        end function
+        ! deriv (subrutinas.f90:88-126)
+        integer(c_int) function cfd_deriv(m, x, y, area, dnx, dny, hh, hhx, hhy, hmin) &
+                bind(c, name='cfd_deriv')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(in) :: x(*), y(*)
+            real(c_double), intent(out) :: area(*), dnx(*), dny(*), hh(*), hhx(*), hhy(*)
+            real(c_double), intent(out) :: hmin
+        end function
+        ! MASAS (subrutinas.f90:128-153)
+        integer(c_int) function cfd_masas(m, area, mass) bind(c, name='cfd_masas')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(in) :: area(*)
+            real(c_double), intent(out) :: mass(*)
+        end function
+        ! deltat (subrutinas.f90:155-218)
+        integer(c_int) function cfd_deltat(m, c, area, t, vel_x, vel_y, w_x, w_y, dt, dtmin) &
+                bind(c, name='cfd_deltat')
+            import :: c_int, c_ptr, c_double, cfd_consts
+            type(c_ptr), value :: m
+            type(cfd_consts), intent(in) :: c
+            real(c_double), intent(in) :: area(*), t(*), vel_x(*), vel_y(*), w_x(*), w_y(*)
+            real(c_double), intent(out) :: dt(*), dtmin
+        end function
+        ! ESTAB (subrutinas.f90:332-446)
+        integer(c_int) function cfd_estab(m, c, u, t, gamm, vel_x, vel_y, w_x, w_y, dnx, dny, &
+                dtmin, shoc, t_sugn1, t_sugn2, t_sugn3) bind(c, name='cfd_estab')
+            import :: c_int, c_ptr, c_double, cfd_consts
+            type(c_ptr), value :: m
+            type(cfd_consts), intent(in) :: c
+            real(c_double), intent(in) :: u(*), t(*), gamm(*), vel_x(*), vel_y(*), w_x(*), w_y(*)
+            real(c_double), intent(in) :: dnx(*), dny(*)
+            real(c_double), value :: dtmin
+            real(c_double), intent(out) :: shoc(*), t_sugn1(*), t_sugn2(*), t_sugn3(*)
+        end function
+        ! calcRHS_mod::calcRHS (calcRHS.f90:4-154): accumulates into rhs
+        integer(c_int) function cfd_calcrhs(m, c, rhs, u, theta, t, dnx, dny, area, shoc, dtl, &
+                t_sugn1, t_sugn2, t_sugn3) bind(c, name='cfd_calcrhs')
+            import :: c_int, c_ptr, c_double, cfd_consts
+            type(c_ptr), value :: m
+            type(cfd_consts), intent(in) :: c
+            real(c_double), intent(inout) :: rhs(*)
+            real(c_double), intent(in) :: u(*), theta(*), t(*), dnx(*), dny(*), area(*)
+            real(c_double), intent(in) :: shoc(*), dtl(*), t_sugn1(*), t_sugn2(*), t_sugn3(*)
+        end function
+        ! FUENTE (subrutinas.f90:1036-1094): accumulates into rhs
+        integer(c_int) function cfd_fuente(m, rhs, u, w_x, w_y, dnx, dny, area, dtl) &
+                bind(c, name='cfd_fuente')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(inout) :: rhs(*)
+            real(c_double), intent(in) :: u(*), w_x(*), w_y(*), dnx(*), dny(*), area(*), dtl(*)
+        end function
+        ! Mnormales::normales (subrutinas.f90:7-64): normals kept for cfd_rk
+        integer(c_int) function cfd_normales(m, bc, x, y) bind(c, name='cfd_normales')
+            import :: c_int, c_ptr, c_double, cfd_bc_lists
+            type(c_ptr), value :: m
+            type(cfd_bc_lists), intent(in) :: bc
+            real(c_double), intent(in) :: x(*), y(*)
+        end function
+        ! RK (subrutinas.f90:645-849, NGAS = 0)
+        integer(c_int) function cfd_rk(m, c, bc, nrk, dtmin, dtl, nodes, elems) &
+                bind(c, name='cfd_rk')
+            import :: c_int, c_ptr, c_double, cfd_consts, cfd_bc_lists, cfd_rk_nodes, cfd_rk_elems
+            type(c_ptr), value :: m
+            type(cfd_consts), intent(in) :: c
+            type(cfd_bc_lists), intent(in) :: bc
+            integer(c_int), value :: nrk
+            real(c_double), value :: dtmin
+            real(c_double), intent(in) :: dtl(*)
+            type(cfd_rk_nodes), intent(in) :: nodes
+            type(cfd_rk_elems), intent(in) :: elems
+        end function
+        ! BiconjGrad::biCG (biconjGrad.f90:8-62)
+        integer(c_int) function cfd_bicg(npoin, spmtx, spidx, sprowptr, diagmtx, x, b, x_fix, &
+                x_fixidx, nfix, iters, rz) bind(c, name='cfd_bicg')
+            import :: c_int, c_double
+            integer(c_int), value :: npoin, nfix
+            real(c_double), intent(in) :: spmtx(*), diagmtx(*), b(*), x_fix(*)
+            integer(c_int), intent(in) :: spidx(*), sprowptr(*), x_fixidx(*)
+            real(c_double), intent(inout) :: x(*)
+            integer(c_int), intent(out) :: iters
+            real(c_double), intent(out) :: rz
+        end function
+        ! gcl_mod (gcl.f90:8-62)
+        integer(c_int) function cfd_gcl_put_w(m, w_x, w_y) bind(c, name='cfd_gcl_put_w')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(in) :: w_x(*), w_y(*)
+        end function
+        integer(c_int) function cfd_gcl_put_area(m, area) bind(c, name='cfd_gcl_put_area')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(in) :: area(*)
+        end function
+        integer(c_int) function cfd_gcl_main(m, mass, w_x, w_y, dnx, dny, area, dt, mode) &
+                bind(c, name='cfd_gcl_main')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(inout) :: mass(*)
+            real(c_double), intent(in) :: w_x(*), w_y(*), dnx(*), dny(*), area(*)
+            real(c_double), value :: dt
+            integer(c_int), value :: mode
+        end function
+    end interface
+
+contains
+
+    ! ERROR STOP with the library's message when code /= 0 (the reference has
+    ! no status codes: it STOPs on fatal conditions).
+    subroutine cfd_check(code, what)
+        integer(c_int), intent(in) :: code
+        character(*), intent(in) :: what
+        character(kind=c_char), pointer :: msg(:)
+        character(len=512) :: text
+        integer :: i
+        if (code == 0) return
+        text = ''
+        call c_f_pointer(cfd_last_error(), msg, [512])
+        do i = 1, 512
+            if (msg(i) == c_null_char) exit
+            text(i:i) = msg(i)
+        end do
+        write (*, '(a,a,a,i0,a,a)') 'cfd: ', what, ' failed (code ', code, '): ', trim(text)
+        error stop 1
+    end subroutine cfd_check
+
+end module cfd_abi

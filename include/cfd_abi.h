@@ -138,6 +138,14 @@ int cfd_rk(cfd_mesh *m, const cfd_consts *c, const cfd_bc_lists *bc, int nrk, do
 int cfd_bicg(int npoin, const double *spMtx, const int *spIdx, const int *spRowptr,
              const double *diagMtx, double *x, const double *b, const double *x_fix,
              const int *x_fixIdx, int nfix, int *iters, double *rz);
+/* cfd_bicg with the stopping rule as arguments (no counterpart in the
+ * reference, which hard-codes tol = 1e-10 and 1000 iterations): rel = 0 is
+ * the reference's absolute rule, rel = 1 the relative rule of
+ * cfd_case_desc.pcg_rel; *converged is set when the tolerance was met. */
+int cfd_bicg_ex(int npoin, const double *spMtx, const int *spIdx, const int *spRowptr,
+                const double *diagMtx, double *x, const double *b, const double *x_fix,
+                const int *x_fixIdx, int nfix, double tol, int maxit, int rel, int *iters,
+                double *rz, int *converged);
 
 /* gcl_mod (gcl.f90:8-62): putW / putArea store the previous mesh velocity and
  * areas in the mesh context; main updates M in place.  mode 0 reproduces the
@@ -165,8 +173,11 @@ typedef struct {
     int motion;              /* 0: DISN(2) = pi/8 sin(10 t) (meshMove.f90:70)
                                 1: alpha(t) = motion_amp * sin(motion_omega t) */
     double motion_amp, motion_omega;
-    int pcg_rel;             /* 0: |r.z| <= pcg_tol absolute (reference)
-                                1: |r.z| <= pcg_tol^2 * |r0.z0| of a zero guess */
+    int pcg_rel;             /* 0: r.r < pcg_tol and |r.z| <= pcg_tol absolute
+                                   (reference)
+                                1: the same tests against pcg_tol^2 times the
+                                   r.r and r.z of the guess that is zero off the
+                                   fixed nodes (independent of the warm start) */
     double pcg_tol;          /* 0 means 1e-10                                 */
     int pcg_maxit;           /* 0 means 1000                                  */
     int gcl;                 /* -1: no GCL (reference); 0/1: gcl mode          */
@@ -199,9 +210,9 @@ int cfd_session_destroy(cfd_session *s);
  * geometry (NORMALES, DERIV, MASAS, laplace: :88-100) */
 int cfd_session_init_freestream(cfd_session *s);
 /* nsteps time steps; returns after the work is queued (cfd_session_sync
- * waits).  The step's scalars (DTMIN, TIME, forces, ...) stay on the device;
- * a fixed-mesh step makes no host round trip.  A moving mesh waits for each
- * mesh solve's convergence test. */
+ * waits).  The step's scalars (DTMIN, TIME, forces, PCG counts, ...) stay on
+ * the device: a fixed-mesh step makes no host round trip; with moving nodes
+ * the host waits for each mesh solve's convergence test. */
 int cfd_session_step(cfd_session *s, int nsteps);
 int cfd_session_sync(cfd_session *s);
 /* field size: npoin, 4*npoin, nelem, 3*nelem or nnz doubles */
