@@ -29,12 +29,21 @@ METRIC = "triangle-RHS evals/s, %HBM roofline; ALE step time learned vs plain at
 
 
 def element_bytes(nelem: int, npoin: int, precision: int = 0) -> int:
-    """Algorithmic bytes of one element-kernel launch (calcRHS element part),
-    each input counted once at the width the variant reads:
-      FP64: per element inpoel 12 + dNx,dNy 48 + area,dtl,shoc,tau1..3 48 = 108 B
-      FP32: per element inpoel 12 + dNx,dNy 24 + fac,nu,tau1..3 20 = 56 B
-      per node U 32 + T 8 = 40 B (theta == 0 inside RK is not read)."""
-    return (108 if precision == 0 else 56) * nelem + 40 * npoin
+    """Algorithmic bytes of one launch of the timed element kernel, each array
+    counted once at the width the variant reads or writes.
+
+    FP64 (k_element_rhs, calcRHS element part): per element inpoel 12 +
+    dNx,dNy 48 + area,dtl,shoc,tau1..3 48 = 108 B; per node U 32 + T 8 = 40 B
+    (theta == 0 inside RK is not read).
+
+    FP32 (k_patch_stage + k_halo_stage: residual, node sums and the RK stage
+    fused): per element inpoel 12 + dNx,dNy 24 + fac,nu,tau1..3 20 = 56 B;
+    per node U 32 + T 8 + M 8 + GAMM 8 = 56 B read, and written on average
+    over the 4 stages (3 x T 8 + final U1 32, RHS 32, T, VX, VY, RHO, E, P,
+    RMACH 56) / 4 = 36 B."""
+    if precision == 0:
+        return 108 * nelem + 40 * npoin
+    return 56 * nelem + 92 * npoin
 
 
 def cpu_baseline(cs, seconds_budget: float = 20.0) -> dict:
@@ -139,12 +148,14 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_element_rhs (calcRHS element residual)",
+            "kernel": ("k_element_rhs (calcRHS element residual)" if prec == 0 else
+                       "k_patch_stage + k_halo_stage (FP32 residual, node sums, RK stage)"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (read_traffic(f"k_element_rhs_{args.precision}_cylinder1024x2048_traffic.txt")
+            "traffic": (read_traffic(("k_element_rhs_fp64" if prec == 0 else "k_patch_stage_fp32")
+                                     + "_cylinder1024x2048_traffic.txt")
                         if (args.nr, args.nt) == (1024, 2048) else None),
             "algorithmic_bytes_per_launch": bytes_elem,
             "avg_launch_us": avg_elem_s * 1e6,

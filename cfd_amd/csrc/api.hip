@@ -701,7 +701,9 @@ struct cfd_session {
     DBuf<double> DTL, DT, SHOC, T1, T2, T3, AREA, DNX, DNY, HH, HHX, HHY, AREA_OLD, WXO, WYO;
     DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, POSAUY, BZERO, partials;
     // FP32 element path: binary32 shape gradients, ESTAB outputs, slots
-    DBuf<float> DNX32, DNY32, NU32, TA32, TB32, TC32, FAC32, c32, f32b;
+    // (all in patch order)
+    DBuf<float> DNX32, DNY32, NU32, TA32, TB32, TC32, FAC32;
+    PatchSet patches;
     DBuf<int> ILAUX, ISET;
     BcDevice bc;
     PcgWork pcg, pcg_y;  // the x and y mesh solves
@@ -784,11 +786,27 @@ struct cfd_session {
         launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
     }
 
-    // binary32 copies of the shape gradients read by the FP32 element kernel
+    // the FP32 path's ESTAB outputs (empty on the FP64 path)
+    Elem32 elem32() {
+        Elem32 o;
+        if (d.precision != 1) return o;
+        o.nu = NU32.p;
+        o.ta = TA32.p;
+        o.tb = TB32.p;
+        o.tc = TC32.p;
+        o.fac = FAC32.p;
+        o.area = AREA.p;
+        o.dtl = DTL.p;
+        o.perm = patches.perm.p;
+        o.cte = c.cte;
+        return o;
+    }
+
+    // binary32 copies of the shape gradients read by the FP32 kernel, in patch order
     void refresh_f32_geometry() {
         if (d.precision != 1) return;
-        launch_to_f32(DNX.p, DNX32.p, 3LL * ne, s);
-        launch_to_f32(DNY.p, DNY32.p, 3LL * ne, s);
+        launch_to_f32_patch(DNX.p, DNX32.p, patches.perm.p, ne, s);
+        launch_to_f32_patch(DNY.p, DNY32.p, patches.perm.p, ne, s);
     }
 
     // The two biCG mesh solves of fluidStructure (meshMove.f90:91-126), then
@@ -834,17 +852,7 @@ struct cfd_session {
 
         // RK (subrutinas.f90:645-849)
         const bool f32 = d.precision == 1;
-        Elem32 o32;
-        if (f32) {
-            o32.nu = NU32.p;
-            o32.ta = TA32.p;
-            o32.tb = TB32.p;
-            o32.tc = TC32.p;
-            o32.fac = FAC32.p;
-            o32.area = AREA.p;
-            o32.dtl = DTL.p;
-            o32.cte = c.cte;
-        }
+        const Elem32 o32 = elem32();
         tb(CFD_K_ESTAB);
         launch_estab(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin, SHOC.p,
                      T1.p, T2.p, T3.p, s, o32);
@@ -852,26 +860,25 @@ struct cfd_session {
         const bool fu = fuente_needed();
         ElemIn in{U(), nullptr, T.p, DNX.p, DNY.p, AREA.p, SHOC.p, DTL.p, T1.p, T2.p, T3.p,
                   fu ? WX.p : nullptr, fu ? WY.p : nullptr};
-        ElemIn32 in32{U(), T.p, DNX32.p, DNY32.p, FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p,
-                      fu ? WX.p : nullptr, fu ? WY.p : nullptr};
+        PatchIn32 in32{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DNX32.p, DNY32.p,
+                       FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
         StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p, WX.p,
                       WY.p};
         const int nrk = d.nrk > 0 ? d.nrk : 4;
         for (int irk = 1; irk <= nrk; ++irk) {
             double rk_fact = 1.0 / (nrk + 1 - irk);
+            if (f32) {  // residual + gather + stage fused (timed as the element kernel)
+                tb(CFD_K_ELEMENT);
+                launch_patch_stage(t, patches, c, bc.tab, rk_fact, in32, sn, irk == nrk, s);
+                te(CFD_K_ELEMENT);
+                continue;
+            }
             tb(CFD_K_ELEMENT);
-            if (f32)
-                launch_element_rhs32(t, c, in32, c32.p, fu ? f32b.p : nullptr, s);
-            else
-                launch_element_rhs(t, c, in, c_calc.p, fu ? c_fuente.p : nullptr, s);
+            launch_element_rhs(t, c, in, c_calc.p, fu ? c_fuente.p : nullptr, s);
             te(CFD_K_ELEMENT);
             tb(CFD_K_STAGE);
-            if (f32)
-                launch_stage_update32(t, c, bc.tab, rk_fact, c32.p, fu ? f32b.p : nullptr, sn,
-                                      irk == nrk, s);
-            else
-                launch_stage_update(t, c, bc.tab, rk_fact, c_calc.p, fu ? c_fuente.p : nullptr, sn,
-                                    irk == nrk, s);
+            launch_stage_update(t, c, bc.tab, rk_fact, c_calc.p, fu ? c_fuente.p : nullptr, sn,
+                                irk == nrk, s);
             te(CFD_K_STAGE);
         }
 
@@ -1003,8 +1010,10 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         if (d->precision == 1) {
             for (auto* b : {&S->DNX32, &S->DNY32}) b->alloc(3 * (size_t)ne);
             for (auto* b : {&S->NU32, &S->TA32, &S->TB32, &S->TC32, &S->FAC32}) b->alloc(ne);
-            S->c32.alloc(12 * (size_t)ne);
-            S->f32b.alloc(12 * (size_t)ne);
+            std::vector<int> inp(3 * (size_t)ne);
+            m->t.inpoel.download(inp.data(), inp.size(), s);
+            CFD_HIP(hipStreamSynchronize(s));
+            build_patches(S->patches, inp.data(), d->X, d->Y, ne, np, s);
         } else {
             S->c_calc.alloc(12 * (size_t)ne);
             S->c_fuente.alloc(12 * (size_t)ne);
@@ -1087,6 +1096,11 @@ int cfd_session_get_field(cfd_session* S, int f, double* host) {
     CFD_TRY({
         long long n = 0;
         DBuf<double>* b = S->field(f, n);
+        const bool estab_field = f == CFD_F_SHOC || f == CFD_F_T_SUGN1 || f == CFD_F_T_SUGN2 ||
+                                 f == CFD_F_T_SUGN3;
+        if (S->d.precision == 1 && estab_field)  // the binary32 values the FP32 path used
+            launch_estab_widen(S->elem32(), S->patches.inv_perm.p, S->ne, S->SHOC.p, S->T1.p,
+                               S->T2.p, S->T3.p, S->s);
         b->download(host, n, S->s);
         CFD_HIP(hipStreamSynchronize(S->s));
     })

@@ -155,11 +155,13 @@ constexpr int kRedBlocks = 2048;
 int launch_deltat_partials(const Tables& t, const Consts& c, const double* area, const double* T,
                            const double* VX, const double* VY, const double* WX, const double* WY,
                            double* DT, double* partials, hipStream_t s);
-// Inputs of the FP32 element residual (nelem floats each), written by ESTAB
-// when nu != nullptr: nu = SHOC*CTE, tau1..3, fac = area*dtl/3.
+// Inputs of the FP32 element residual (nelem floats each, patch order),
+// written by ESTAB instead of its FP64 outputs when nu != nullptr: nu =
+// SHOC*CTE, tau1..3, fac = area*dtl/3; position q holds element perm[q].
 struct Elem32 {
     float *nu = nullptr, *ta = nullptr, *tb = nullptr, *tc = nullptr, *fac = nullptr;
     const double *area = nullptr, *dtl = nullptr;
+    const int* perm = nullptr;
     double cte = 1.0;
 };
 
@@ -169,6 +171,9 @@ void launch_estab(const Tables& t, const Consts& c, const double* U, const doubl
                   const double* WY, const double* dNx, const double* dNy, const double* dtmin,
                   double* shoc, double* t1, double* t2, double* t3, hipStream_t s,
                   const Elem32& o32 = Elem32());
+// SHOC, T_SUGN1..3 (element order, FP64) from the binary32 copies of o32
+void launch_estab_widen(const Elem32& o32, const int* inv_perm, int nelem, double* shoc,
+                        double* t1, double* t2, double* t3, hipStream_t s);
 
 // Element residual: calcRHS (calcRHS.f90:36-151) and, when WX != nullptr,
 // FUENTE (subrutinas.f90:1060-1091), written to per-slot buffers indexed
@@ -181,16 +186,6 @@ void launch_element_rhs(const Tables& t, const Consts& c, const ElemIn& in, doub
 
 // FUENTE alone into its slot buffer (uses U, dNx, dNy, area, dtl, WX, WY)
 void launch_fuente(const Tables& t, const ElemIn& in, double* c_fuente, hipStream_t s);
-
-// FP32 element residual (+ FUENTE when WX != nullptr), float4 slots
-struct ElemIn32 {
-    const double *U, *T;
-    const float *dNx, *dNy, *fac, *nu, *ta, *tb, *tc;
-    const double *WX, *WY;
-};
-void launch_element_rhs32(const Tables& t, const Consts& c, const ElemIn32& in, float* c_calc,
-                          float* c_fuente, hipStream_t s);
-void launch_to_f32(const double* a, float* b, long long n, hipStream_t s);
 
 // Deterministic node gather: rhs(i) = rhs_in(i) + sum calc contributions in
 // esup order, then + sum FUENTE contributions (reference accumulation order).
@@ -219,10 +214,43 @@ struct StageNodes {
 void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
                          const double* c_calc, const double* c_fuente, const StageNodes& nd,
                          bool final_stage, hipStream_t s);
-// the same with FP32 slots (summed in FP64)
-void launch_stage_update32(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
-                           const float* c_calc, const float* c_fuente, const StageNodes& nd,
-                           bool final_stage, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Element patches for the fused FP32 path (patches.hip): Morton-ordered runs
+// of kPatchElems elements; per element (in patch order) one 64-bit word (two
+// uint32) with the local ids of its nodes (3 x 10 bits) and its local slots
+// (3 x 11 bits from bit 30); per patch its nodes,
+// interior ones first, with each node's slot range (lptr); per halo node its
+// partial-sum entries in ascending patch order.
+// ---------------------------------------------------------------------------
+constexpr int kPatchElems = 512;
+constexpr int kPatchMaxNodes = 700;  // keeps the fused kernel's LDS within 64 KB
+struct PatchSet {
+    int npatch = 0, max_nodes = 0, nentries = 0, nhalo = 0;
+    DBuf<int> perm, inv_perm;      // patch position <-> element (0-based)
+    DBuf<uint32_t> loc;            // 2 per element (patch order)
+    DBuf<int> node_base, n_int, halo_base;
+    DBuf<int> pnode;               // 1-based node ids, per patch
+    DBuf<uint16_t> lptr;           // nl+1 per patch, at node_base[p] + p
+    DBuf<int> hnode, hptr, hidx;   // halo nodes (1-based), CSR of their entries
+    DBuf<double> hpart;            // 4 per entry
+};
+void build_patches(PatchSet& ps, const int* inpoel_host, const double* X_host,
+                   const double* Y_host, int nelem, int npoin, hipStream_t s);
+
+// Fused FP32 residual + RK stage for the patch-interior nodes, then the
+// stage for the halo nodes (subrutinas.f90:679-826 with calcRHS.f90:37-141
+// and FUENTE in binary32, node sums in FP64).  Element inputs are in patch
+// order: dNx/dNy (3 floats), fac, nu, tau1..3.
+struct PatchIn32 {
+    const double *U, *T, *WX, *WY;
+    const float *dNx, *dNy, *fac, *nu, *ta, *tb, *tc;
+};
+void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, const BcTable& bc,
+                        double rk_fact, const PatchIn32& in, const StageNodes& nd, bool final_stage,
+                        hipStream_t s);
+// FP32 copies of dNx/dNy in patch order
+void launch_to_f32_patch(const double* a, float* b, const int* perm, int nelem, hipStream_t s);
 
 // NORMALES (subrutinas.f90:7-64) on the BC table entries that are wall nodes
 void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, const int* wnode_ptr,

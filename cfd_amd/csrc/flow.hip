@@ -70,8 +70,11 @@ __global__ void k_deltat_clip(double* __restrict__ DT, const double* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// ESTAB
+// ESTAB.  kF32 = false: the FP64 outputs SHOC, T_SUGN1..3 per element.
+// kF32 = true (FP32 path): thread q handles element perm[q] and writes only
+// the binary32 inputs of the fused residual, at q (patch order).
 // ---------------------------------------------------------------------------
+template <bool kF32>
 __global__ void __launch_bounds__(kBlock)
 k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const double* __restrict__ T,
         const double* __restrict__ GAMM, const double* __restrict__ VEL_X,
@@ -80,8 +83,9 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
         const double* __restrict__ dNy, int nelem, double fr, double tinf, double rhoinf,
         const double* __restrict__ dtminp, double* __restrict__ shoc, double* __restrict__ t1,
         double* __restrict__ t2, double* __restrict__ t3, Elem32 o32) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nelem) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nelem) return;
+    const int e = kF32 ? o32.perm[q] : q;
     const double dtmin = *dtminp;
     int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
     double DNX[3] = {dNx[3 * e], dNx[3 * e + 1], dNx[3 * e + 2]};
@@ -135,7 +139,7 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
     if (h_jgn > 10.0) h_jgn = 0.0;
     double tr1 = dr2 * h_jgn / rho_elem;
     double zzz = h_jgn / (2.0 * cc);
-    shoc[e] = (tr1 + tr1 * tr1) * 0.5 * (cc * cc) * zzz;
+    const double sh = (tr1 + tr1 * tr1) * 0.5 * (cc * cc) * zzz;
     double resumen = 1.0 / (tau * tau) + (2.0 / dtmin) * (2.0 / dtmin);
     double rrr = pow_m1_2(resumen);
     double s2 = rrr, s3 = rrr;
@@ -145,17 +149,34 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
         s2 = pow_m1_2(resumen + 1.0 / (ts3 * ts3));
         s3 = pow_m1_2(resumen + 1.0 / (ts3e * ts3e));
     }
-    t1[e] = rrr;
-    t2[e] = s2;
-    t3[e] = s3;
-    if (o32.nu) {  // inputs of the FP32 element residual, rounded once here
-        double sh = (tr1 + tr1 * tr1) * 0.5 * (cc * cc) * zzz;
-        o32.nu[e] = (float)(sh * o32.cte);
-        o32.ta[e] = (float)rrr;
-        o32.tb[e] = (float)s2;
-        o32.tc[e] = (float)s3;
-        o32.fac[e] = (float)(o32.area[e] * o32.dtl[e] / 3.0);
+    if (kF32) {  // inputs of the FP32 element residual, rounded once here
+        o32.nu[q] = (float)(sh * o32.cte);
+        o32.ta[q] = (float)rrr;
+        o32.tb[q] = (float)s2;
+        o32.tc[q] = (float)s3;
+        o32.fac[q] = (float)(o32.area[e] * o32.dtl[e] / 3.0);
+    } else {
+        shoc[e] = sh;
+        t1[e] = rrr;
+        t2[e] = s2;
+        t3[e] = s3;
     }
+}
+
+// SHOC, T_SUGN1..3 from the binary32 copies (FP32 path): element e's values
+// sit at patch position inv_perm[e]
+__global__ void k_estab_widen(const float* __restrict__ nu, const float* __restrict__ ta,
+                              const float* __restrict__ tb, const float* __restrict__ tc,
+                              const int* __restrict__ inv_perm, double inv_cte, int nelem,
+                              double* __restrict__ shoc, double* __restrict__ t1,
+                              double* __restrict__ t2, double* __restrict__ t3) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    const int q = inv_perm[e];
+    shoc[e] = (double)nu[q] * inv_cte;
+    t1[e] = ta[q];
+    t2[e] = tb[q];
+    t3[e] = tc[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -432,21 +453,71 @@ k_gather_rhs(const int* __restrict__ esup2, const double* rhs_in, const CT* __re
     if (act) st4(rhs_out + 4 * (long long)i, r0, r1, r2, r3);
 }
 
-// Gather + one RK stage for one node (subrutinas.f90:679-826, NGAS = 0).
-// kFinal = false: only T is stored (the next stage's only nodal input besides
-// U); the final stage stores everything.
-template <class CT, bool kFuente, bool kFinal>
+// One RK stage for one node, given its summed residual (subrutinas.f90:679-826,
+// NGAS = 0).  kFinal = false: only T is stored (the next stage's only nodal
+// input besides U); the final stage stores everything.
+struct StageDev {
+    double rk_fact, fr;
+    const double *M, *GAMM, *WX, *WY;
+    const int *bcidx, *bcflags;
+    const double* bcvals;
+    double *U1, *RHS, *T, *VX, *VY, *RHO, *E, *P, *RMACH;
+};
+
+template <bool kFinal>
+__device__ inline void stage_node(const StageDev& a, int i, double4 u, double r0, double r1,
+                                  double r2, double r3) {
+    if (kFinal && a.RHS) st4(a.RHS + 4 * (long long)i, r0, r1, r2, r3);
+    double f = a.rk_fact / a.M[i];
+    double u0 = u.x - f * r0, u1 = u.y - f * r1, u2 = u.z - f * r2, u3 = u.w - f * r3;
+    double gamm = a.GAMM[i];
+    double rho = u0;
+    double vx = u1 / rho;
+    double vy = u2 / rho;
+    double en = u3 / rho;
+    double vel2 = vx * vx + vy * vy;
+    double pr = rho * (gamm - 1.0) * (en - 0.5 * vel2);
+    double tt = pr / (rho * a.fr);
+    int bi = a.bcidx[i];
+    if (!kFinal) {  // T after the BCs: only FIX T changes it
+        if (bi >= 0 && (a.bcflags[bi] & 8)) tt = a.bcvals[8 * bi + 5];
+        a.T[i] = tt;
+        return;
+    }
+    a.P[i] = pr;
+    a.RMACH[i] = sqrt(vel2 / (tt * gamm * a.fr));
+    if (bi >= 0) {
+        int fl = a.bcflags[bi];
+        const double* v = a.bcvals + 8 * bi;
+        if (fl & 1) { vx = v[0]; vy = v[1]; }                 // FIXVEL
+        if (fl & 2) {                                          // NORMALVEL
+            double wx = a.WX[i], wy = a.WY[i], nx = v[2], ny = v[3];
+            double pp = -ny * (vx - wx) + nx * (vy - wy);
+            vx = -ny * pp + wx;
+            vy = nx * pp + wy;
+        }
+        if (fl & 4) rho = v[4];                                // FIX rho
+        if (fl & 8) {                                          // FIX T
+            double gm = gamm - 1.0;
+            tt = v[5];
+            en = tt * a.fr / gm + 0.5 * (vx * vx + vy * vy);
+        }
+    }
+    a.RHO[i] = rho;
+    a.VX[i] = vx;
+    a.VY[i] = vy;
+    a.E[i] = en;
+    a.T[i] = tt;
+    st4(a.U1 + 4 * (long long)i, rho, vx * rho, vy * rho, en * rho);
+}
+
+// Gather (calcRHS slots, then FUENTE slots, in esup order) + one RK stage
+template <bool kFuente, bool kFinal>
 __global__ void __launch_bounds__(kBlock)
-k_stage(const int* __restrict__ esup2, const CT* __restrict__ c_calc,
-        const CT* __restrict__ c_fuente, double rk_fact, double fr,
-        const double* __restrict__ U, const double* __restrict__ M, const double* __restrict__ GAMM,
-        const double* __restrict__ WX, const double* __restrict__ WY,
-        const int* __restrict__ bcidx, const int* __restrict__ bcflags,
-        const double* __restrict__ bcvals, double* __restrict__ U1, double* __restrict__ RHS,
-        double* __restrict__ T, double* __restrict__ VX, double* __restrict__ VY,
-        double* __restrict__ RHO, double* __restrict__ E, double* __restrict__ P,
-        double* __restrict__ RMACH, int npoin) {
-    __shared__ CT sbuf[kChunk];
+k_stage(const int* __restrict__ esup2, const double4* __restrict__ c_calc,
+        const double4* __restrict__ c_fuente, const double* __restrict__ U, StageDev a,
+        int npoin) {
+    __shared__ double4 sbuf[kChunk];
     const int i0 = blockIdx.x * kBlock, i1 = min(i0 + kBlock, npoin);
     const int i = i0 + threadIdx.x;
     const bool act = i < npoin;
@@ -456,49 +527,7 @@ k_stage(const int* __restrict__ esup2, const CT* __restrict__ c_calc,
     staged_sum(c_calc, s0, s1, b, e, sbuf, r0, r1, r2, r3);
     if (kFuente) staged_sum(c_fuente, s0, s1, b, e, sbuf, r0, r1, r2, r3);
     if (!act) return;
-    if (kFinal && RHS) st4(RHS + 4 * (long long)i, r0, r1, r2, r3);
-    double4 u = ld4(U + 4 * (long long)i);
-    double f = rk_fact / M[i];
-    double u0 = u.x - f * r0, u1 = u.y - f * r1, u2 = u.z - f * r2, u3 = u.w - f * r3;
-    double gamm = GAMM[i];
-    double rho = u0;
-    double vx = u1 / rho;
-    double vy = u2 / rho;
-    double en = u3 / rho;
-    double vel2 = vx * vx + vy * vy;
-    double pr = rho * (gamm - 1.0) * (en - 0.5 * vel2);
-    double tt = pr / (rho * fr);
-    int bi = bcidx[i];
-    if (!kFinal) {  // T after the BCs: only FIX T changes it
-        if (bi >= 0 && (bcflags[bi] & 8)) tt = bcvals[8 * bi + 5];
-        T[i] = tt;
-        return;
-    }
-    P[i] = pr;
-    RMACH[i] = sqrt(vel2 / (tt * gamm * fr));
-    if (bi >= 0) {
-        int fl = bcflags[bi];
-        const double* v = bcvals + 8 * bi;
-        if (fl & 1) { vx = v[0]; vy = v[1]; }                 // FIXVEL
-        if (fl & 2) {                                          // NORMALVEL
-            double wx = WX[i], wy = WY[i], nx = v[2], ny = v[3];
-            double pp = -ny * (vx - wx) + nx * (vy - wy);
-            vx = -ny * pp + wx;
-            vy = nx * pp + wy;
-        }
-        if (fl & 4) rho = v[4];                                // FIX rho
-        if (fl & 8) {                                          // FIX T
-            double gm = gamm - 1.0;
-            tt = v[5];
-            en = tt * fr / gm + 0.5 * (vx * vx + vy * vy);
-        }
-    }
-    RHO[i] = rho;
-    VX[i] = vx;
-    VY[i] = vy;
-    E[i] = en;
-    T[i] = tt;
-    st4(U1 + 4 * (long long)i, rho, vx * rho, vy * rho, en * rho);
+    stage_node<kFinal>(a, i, ld4(U + 4 * (long long)i), r0, r1, r2, r3);
 }
 
 // ---------------------------------------------------------------------------
@@ -521,38 +550,18 @@ struct ElemConsts32 {
 
 __device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// One element's FP32 residual at its three nodes, calcRHS plus (kFuente)
+// FUENTE, into out[j] (node j of the element).  un/T/WX/WY: the FP64 node
+// values; Nx, Ny: binary32 shape gradients; ar = area*dtl/3; nu, tau from
+// ESTAB.
 template <bool kFuente>
-__global__ void __launch_bounds__(kBlock)
-k_element_rhs32(ElemConsts32 kc, const int* __restrict__ inpoel, const int* __restrict__ epos,
-                const double* __restrict__ U, const double* __restrict__ T,
-                const float* __restrict__ dNx, const float* __restrict__ dNy,
-                const float* __restrict__ fac, const float* __restrict__ nu_,
-                const float* __restrict__ ta, const float* __restrict__ tb,
-                const float* __restrict__ tc, const double* __restrict__ WX,
-                const double* __restrict__ WY, float4* __restrict__ c_calc,
-                float4* __restrict__ c_fuente, int nelem) {
+__device__ inline void elem32_residual(const ElemConsts32& kc, const double4 u0, const double4 u1,
+                                       const double4 u2, const double (&T)[3],
+                                       const double (&WX)[3], const double (&WY)[3],
+                                       const float (&Nx)[3], const float (&Ny)[3], const float ar,
+                                       const float nu, const float (&tau)[3], float4 (&out)[3]) {
 #pragma clang fp contract(fast)
-    __shared__ int s_in[3 * kBlock];
-    __shared__ int s_pos[3 * kBlock];
-    __shared__ float s_nx[3 * kBlock];
-    __shared__ float s_ny[3 * kBlock];
-    const int e0 = blockIdx.x * kBlock;
-    const int nloc = min(kBlock, nelem - e0);
-    for (int q = threadIdx.x; q < 3 * nloc; q += kBlock) {
-        s_in[q] = inpoel[3 * (long long)e0 + q];
-        s_pos[q] = epos[3 * (long long)e0 + q];
-        s_nx[q] = dNx[3 * (long long)e0 + q];
-        s_ny[q] = dNy[3 * (long long)e0 + q];
-    }
-    __syncthreads();
-    if ((int)threadIdx.x >= nloc) return;
-    const int e = e0 + threadIdx.x;
-    const int l = 3 * threadIdx.x;
-    const int p0 = s_in[l] - 1, p1 = s_in[l + 1] - 1, p2 = s_in[l + 2] - 1;
-    const float Nx[3] = {s_nx[l], s_nx[l + 1], s_nx[l + 2]};
-    const float Ny[3] = {s_ny[l], s_ny[l + 1], s_ny[l + 2]};
-    const double4 u0 = ld4(U + 4 * (long long)p0), u1 = ld4(U + 4 * (long long)p1),
-                  u2 = ld4(U + 4 * (long long)p2);
+    const int p0 = 0, p1 = 1, p2 = 2;
     const double a0[4] = {u0.x, u0.y, u0.z, u0.w};
     const double a1[4] = {u1.x, u1.y, u1.z, u1.w};
     const double a2[4] = {u2.x, u2.y, u2.z, u2.w};
@@ -566,9 +575,6 @@ k_element_rhs32(ElemConsts32 kc, const int* __restrict__ inpoel, const int* __re
         Q[1][q] = (float)(0.5 * (a0[q] + a2[q]));
         Q[2][q] = (float)(0.5 * (a0[q] + a1[q]));
     }
-    const float ar = fac[e];
-    const float tau[3] = {ta[e], tb[e], tc[e]};
-    const float nu = nu_[e];
     const float g = kc.g, gm1 = g - 1.0f, Cv = kc.Cv;
     float mu = 0.0f, lambda = 0.0f;
     if (kc.viscous) {
@@ -625,19 +631,17 @@ k_element_rhs32(ElemConsts32 kc, const int* __restrict__ inpoel, const int* __re
                       3.0f * Uy[3] * lambda) * ir3cv;
         }
     }
-    const int pos[3] = {s_pos[l], s_pos[l + 1], s_pos[l + 2]};
+    float r[3][4];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        float r[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             float gal = 0.5f * ((A[0][q] + A[1][q] + A[2][q]) - A[j][q]);
             float supg = tau[j] * (Nx[j] * S1[q] + Ny[j] * S2[q]);
             float shk = 3.0f * nu * (Nx[j] * Ux[q] + Ny[j] * Uy[q]);
             float vis = Nx[j] * V1[q] + Ny[j] * V2[q];
-            r[q] = ar * (gal + supg + shk + vis);
+            r[j][q] = ar * (gal + supg + shk + vis);
         }
-        c_calc[pos[j]] = make_float4(r[0], r[1], r[2], r[3]);
     }
     if (kFuente) {
         // subrutinas.f90:1064-1078: -AR sum_k sp(j,k) (Ux wx_k + Uy wy_k)
@@ -649,22 +653,121 @@ k_element_rhs32(ElemConsts32 kc, const int* __restrict__ inpoel, const int* __re
         const float sp[3][3] = {{0.5f, 0.0f, 0.5f}, {0.5f, 0.5f, 0.0f}, {0.0f, 0.5f, 0.5f}};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            float r[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 float acc = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) acc += sp[j][k] * (Ux[q] * wx[k] + Uy[q] * wy[k]);
-                r[q] = -ar * acc;
+                r[j][q] += -ar * acc;
             }
-            c_fuente[pos[j]] = make_float4(r[0], r[1], r[2], r[3]);
         }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) out[j] = make_float4(r[j][0], r[j][1], r[j][2], r[j][3]);
+}
+
+// Fused FP32 residual + stage, one patch per block: node values to LDS; each
+// thread computes elements of the patch into LDS slots; each node sums its
+// slots in FP64, then interior nodes take the RK stage and halo nodes store
+// their partial sum.
+struct PatchDev {
+    const uint32_t* loc;
+    const int *node_base, *n_int, *halo_base, *pnode;
+    const uint16_t* lptr;
+    double* hpart;
+    int nelem, maxn;
+};
+
+template <bool kFuente, bool kFinal>
+__global__ void __launch_bounds__(kBlock)
+k_patch_stage(PatchDev pd, ElemConsts32 kc, PatchIn32 in, StageDev a) {
+    extern __shared__ double4 smem4[];
+    const int maxn = pd.maxn;
+    double4* sU = smem4;                                                // maxn
+    double2* sW = reinterpret_cast<double2*>(sU + maxn);                // maxn (kFuente)
+    float4* sC = reinterpret_cast<float4*>(sW + (kFuente ? maxn : 0));  // 3*kPatchElems
+    double* sT = reinterpret_cast<double*>(sC + 3 * kPatchElems);       // maxn
+    uint16_t* sP = reinterpret_cast<uint16_t*>(sT + maxn);              // maxn+1
+    const int p = blockIdx.x;
+    const int nb = pd.node_base[p], nl = pd.node_base[p + 1] - nb, nint = pd.n_int[p];
+    const int e0 = p * kPatchElems, nep = min(kPatchElems, pd.nelem - e0);
+    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+        const int g = pd.pnode[nb + l] - 1;
+        sU[l] = ld4(in.U + 4 * (long long)g);
+        sT[l] = in.T[g];
+        if (kFuente) sW[l] = make_double2(in.WX[g], in.WY[g]);
+    }
+    for (int l = threadIdx.x; l <= nl; l += blockDim.x) sP[l] = pd.lptr[nb + p + l];
+    __syncthreads();
+    for (int q = threadIdx.x; q < nep; q += blockDim.x) {
+        const long long e = e0 + q;
+        const uint64_t w = (uint64_t)pd.loc[2 * e] | ((uint64_t)pd.loc[2 * e + 1] << 32);
+        const int ln[3] = {(int)(w & 1023u), (int)((w >> 10) & 1023u), (int)((w >> 20) & 1023u)};
+        const int ls[3] = {(int)((w >> 30) & 2047u), (int)((w >> 41) & 2047u),
+                           (int)((w >> 52) & 2047u)};
+        const float Nx[3] = {in.dNx[3 * e], in.dNx[3 * e + 1], in.dNx[3 * e + 2]};
+        const float Ny[3] = {in.dNy[3 * e], in.dNy[3 * e + 1], in.dNy[3 * e + 2]};
+        const float tau[3] = {in.ta[e], in.tb[e], in.tc[e]};
+        const double T3[3] = {sT[ln[0]], sT[ln[1]], sT[ln[2]]};
+        double wx3[3] = {0.0, 0.0, 0.0}, wy3[3] = {0.0, 0.0, 0.0};
+        if (kFuente) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                wx3[k] = sW[ln[k]].x;
+                wy3[k] = sW[ln[k]].y;
+            }
+        }
+        float4 out[3];
+        elem32_residual<kFuente>(kc, sU[ln[0]], sU[ln[1]], sU[ln[2]], T3, wx3, wy3, Nx, Ny,
+                                 in.fac[e], in.nu[e], tau, out);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sC[ls[k]] = out[k];
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+        double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+        for (int s = sP[l]; s < sP[l + 1]; ++s) {
+            const float4 c = sC[s];
+            r0 = r0 + (double)c.x;
+            r1 = r1 + (double)c.y;
+            r2 = r2 + (double)c.z;
+            r3 = r3 + (double)c.w;
+        }
+        if (l < nint)
+            stage_node<kFinal>(a, pd.pnode[nb + l] - 1, sU[l], r0, r1, r2, r3);
+        else
+            st4(pd.hpart + 4 * (long long)(pd.halo_base[p] + l - nint), r0, r1, r2, r3);
     }
 }
 
-__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
+// Halo nodes: partial sums in ascending patch order, then the stage
+template <bool kFinal>
+__global__ void __launch_bounds__(kBlock)
+k_halo_stage(const int* __restrict__ hnode, const int* __restrict__ hptr,
+             const int* __restrict__ hidx, const double* __restrict__ hpart,
+             const double* __restrict__ U, StageDev a, int nhalo) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nhalo) return;
+    const int g = hnode[h] - 1;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    for (int j = hptr[h]; j < hptr[h + 1]; ++j) {
+        const double4 v = ld4(hpart + 4 * (long long)hidx[j]);
+        r0 = r0 + v.x;
+        r1 = r1 + v.y;
+        r2 = r2 + v.z;
+        r3 = r3 + v.w;
+    }
+    stage_node<kFinal>(a, g, ld4(U + 4 * (long long)g), r0, r1, r2, r3);
+}
+
+// b[3q+k] = (float)a[3 perm[q] + k]
+__global__ void k_to_f32_patch(const double* __restrict__ a, float* __restrict__ b,
+                               const int* __restrict__ perm, int nelem) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nelem) return;
+    const long long e = perm[q];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) b[3 * (long long)q + k] = (float)a[3 * e + k];
 }
 
 }  // namespace
@@ -694,9 +797,21 @@ void launch_estab(const Tables& t, const Consts& c, const double* U, const doubl
                   const double* WY, const double* dNx, const double* dNy, const double* dtmin,
                   double* shoc, double* t1, double* t2, double* t3, hipStream_t s,
                   const Elem32& o32) {
-    k_estab<<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy,
-                                                 t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin, shoc, t1,
-                                                 t2, t3, o32);
+    if (o32.nu)
+        k_estab<true><<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY,
+                                                           dNx, dNy, t.nelem, c.fr, c.t_inf,
+                                                           c.rho_inf, dtmin, shoc, t1, t2, t3, o32);
+    else
+        k_estab<false><<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY,
+                                                            dNx, dNy, t.nelem, c.fr, c.t_inf,
+                                                            c.rho_inf, dtmin, shoc, t1, t2, t3, o32);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_estab_widen(const Elem32& o32, const int* inv_perm, int nelem, double* shoc,
+                        double* t1, double* t2, double* t3, hipStream_t s) {
+    k_estab_widen<<<grid_for(nelem), kBlock, 0, s>>>(o32.nu, o32.ta, o32.tb, o32.tc, inv_perm,
+                                                     1.0 / o32.cte, nelem, shoc, t1, t2, t3);
     CFD_HIP(hipGetLastError());
 }
 
@@ -745,66 +860,96 @@ void launch_gather_rhs(const Tables& t, const double* rhs_in, const double* c_ca
     CFD_HIP(hipGetLastError());
 }
 
-template <class CT, bool kFuente, bool kFinal>
-static void stage_launch_t(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
-                           const CT* cc, const CT* cf, const StageNodes& nd, hipStream_t s) {
-    k_stage<CT, kFuente, kFinal><<<grid_for(t.npoin), kBlock, 0, s>>>(
-        t.esup2.p, cc, cf, rk_fact, c.fr, nd.U, nd.M, nd.GAMM, nd.WX, nd.WY, bc.bcidx.p,
-        bc.flags.p, bc.vals.p, nd.U1, nd.RHS, nd.T, nd.VX, nd.VY, nd.RHO, nd.E, nd.P, nd.RMACH,
-        t.npoin);
-}
-
-template <class CT>
-static void stage_launch(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
-                         const CT* cc, const CT* cf, const StageNodes& nd, bool final_stage,
-                         hipStream_t s) {
-    if (cf && final_stage) stage_launch_t<CT, true, true>(t, c, bc, rk_fact, cc, cf, nd, s);
-    else if (cf) stage_launch_t<CT, true, false>(t, c, bc, rk_fact, cc, cf, nd, s);
-    else if (final_stage) stage_launch_t<CT, false, true>(t, c, bc, rk_fact, cc, cf, nd, s);
-    else stage_launch_t<CT, false, false>(t, c, bc, rk_fact, cc, cf, nd, s);
-    CFD_HIP(hipGetLastError());
+static StageDev stage_dev(const Consts& c, const BcTable& bc, double rk_fact,
+                          const StageNodes& nd) {
+    StageDev a;
+    a.rk_fact = rk_fact;
+    a.fr = c.fr;
+    a.M = nd.M;
+    a.GAMM = nd.GAMM;
+    a.WX = nd.WX;
+    a.WY = nd.WY;
+    a.bcidx = bc.bcidx.p;
+    a.bcflags = bc.flags.p;
+    a.bcvals = bc.vals.p;
+    a.U1 = nd.U1;
+    a.RHS = nd.RHS;
+    a.T = nd.T;
+    a.VX = nd.VX;
+    a.VY = nd.VY;
+    a.RHO = nd.RHO;
+    a.E = nd.E;
+    a.P = nd.P;
+    a.RMACH = nd.RMACH;
+    return a;
 }
 
 void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
                          const double* c_calc, const double* c_fuente, const StageNodes& nd,
                          bool final_stage, hipStream_t s) {
-    stage_launch(t, c, bc, rk_fact, reinterpret_cast<const double4*>(c_calc),
-                 reinterpret_cast<const double4*>(c_fuente), nd, final_stage, s);
-}
-
-void launch_stage_update32(const Tables& t, const Consts& c, const BcTable& bc, double rk_fact,
-                           const float* c_calc, const float* c_fuente, const StageNodes& nd,
-                           bool final_stage, hipStream_t s) {
-    stage_launch(t, c, bc, rk_fact, reinterpret_cast<const float4*>(c_calc),
-                 reinterpret_cast<const float4*>(c_fuente), nd, final_stage, s);
-}
-
-void launch_element_rhs32(const Tables& t, const Consts& c, const ElemIn32& in, float* c_calc,
-                          float* c_fuente, hipStream_t s) {
-    ElemConsts32 k;
-    k.g = (float)c.gama;
-    k.Cv = (float)c.fcv;
-    k.lambda_ref = (float)c.fk;
-    k.mu_ref = (float)c.fmu;
-    k.T_inf = (float)c.t_inf;
-    k.viscous = c.fmu > 2.2250738585072014e-308 ? 1 : 0;
-    dim3 g(grid_for(t.nelem)), b(kBlock);
-    float4* cc = reinterpret_cast<float4*>(c_calc);
-    float4* cf = reinterpret_cast<float4*>(c_fuente);
-    if (in.WX)
-        k_element_rhs32<true><<<g, b, 0, s>>>(k, t.inpoel.p, t.epos.p, in.U, in.T, in.dNx, in.dNy,
-                                              in.fac, in.nu, in.ta, in.tb, in.tc, in.WX, in.WY, cc, cf,
-                                              t.nelem);
+    const StageDev a = stage_dev(c, bc, rk_fact, nd);
+    const double4* cc = reinterpret_cast<const double4*>(c_calc);
+    const double4* cf = reinterpret_cast<const double4*>(c_fuente);
+    const unsigned g = grid_for(t.npoin);
+    if (cf && final_stage)
+        k_stage<true, true><<<g, kBlock, 0, s>>>(t.esup2.p, cc, cf, nd.U, a, t.npoin);
+    else if (cf)
+        k_stage<true, false><<<g, kBlock, 0, s>>>(t.esup2.p, cc, cf, nd.U, a, t.npoin);
+    else if (final_stage)
+        k_stage<false, true><<<g, kBlock, 0, s>>>(t.esup2.p, cc, cf, nd.U, a, t.npoin);
     else
-        k_element_rhs32<false><<<g, b, 0, s>>>(k, t.inpoel.p, t.epos.p, in.U, in.T, in.dNx, in.dNy,
-                                               in.fac, in.nu, in.ta, in.tb, in.tc, nullptr, nullptr,
-                                               cc, nullptr, t.nelem);
+        k_stage<false, false><<<g, kBlock, 0, s>>>(t.esup2.p, cc, cf, nd.U, a, t.npoin);
     CFD_HIP(hipGetLastError());
 }
 
-void launch_to_f32(const double* a, float* b, long long n, hipStream_t s) {
-    if (n <= 0) return;
-    k_to_f32<<<grid_for((size_t)n), kBlock, 0, s>>>(a, b, n);
+static size_t patch_lds_bytes(int maxn, bool fuente) {
+    return (size_t)maxn * (sizeof(double4) + sizeof(double) + (fuente ? sizeof(double2) : 0)) +
+           3 * (size_t)kPatchElems * sizeof(float4) + (size_t)(maxn + 1) * sizeof(uint16_t);
+}
+
+template <bool kFuente, bool kFinal>
+static void patch_launch_t(const PatchSet& ps, const PatchDev& pd, const ElemConsts32& kc,
+                           const PatchIn32& in, const StageDev& a, hipStream_t s) {
+    const size_t lds = patch_lds_bytes(pd.maxn, kFuente);
+    k_patch_stage<kFuente, kFinal><<<ps.npatch, kBlock, lds, s>>>(pd, kc, in, a);
+    if (ps.nhalo > 0)
+        k_halo_stage<kFinal><<<grid_for(ps.nhalo), kBlock, 0, s>>>(
+            ps.hnode.p, ps.hptr.p, ps.hidx.p, ps.hpart.p, in.U, a, ps.nhalo);
+}
+
+void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, const BcTable& bc,
+                        double rk_fact, const PatchIn32& in, const StageNodes& nd, bool final_stage,
+                        hipStream_t s) {
+    ElemConsts32 kc;
+    kc.g = (float)c.gama;
+    kc.Cv = (float)c.fcv;
+    kc.lambda_ref = (float)c.fk;
+    kc.mu_ref = (float)c.fmu;
+    kc.T_inf = (float)c.t_inf;
+    kc.viscous = c.fmu > 2.2250738585072014e-308 ? 1 : 0;
+    PatchDev pd;
+    pd.loc = ps.loc.p;
+    pd.node_base = ps.node_base.p;
+    pd.n_int = ps.n_int.p;
+    pd.halo_base = ps.halo_base.p;
+    pd.pnode = ps.pnode.p;
+    pd.lptr = ps.lptr.p;
+    pd.hpart = ps.hpart.p;
+    pd.nelem = t.nelem;
+    pd.maxn = ps.max_nodes;
+    CFD_CHECK(patch_lds_bytes(pd.maxn, true) <= 65536, "mesh: element patch exceeds the LDS budget");
+    const StageDev a = stage_dev(c, bc, rk_fact, nd);
+    const bool fu = in.WX != nullptr;
+    if (fu && final_stage) patch_launch_t<true, true>(ps, pd, kc, in, a, s);
+    else if (fu) patch_launch_t<true, false>(ps, pd, kc, in, a, s);
+    else if (final_stage) patch_launch_t<false, true>(ps, pd, kc, in, a, s);
+    else patch_launch_t<false, false>(ps, pd, kc, in, a, s);
+    CFD_HIP(hipGetLastError());
+}
+
+void launch_to_f32_patch(const double* a, float* b, const int* perm, int nelem, hipStream_t s) {
+    if (nelem <= 0) return;
+    k_to_f32_patch<<<grid_for(nelem), kBlock, 0, s>>>(a, b, perm, nelem);
     CFD_HIP(hipGetLastError());
 }
 

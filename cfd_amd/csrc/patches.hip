@@ -1,0 +1,183 @@
+// Element patches for the fused FP32 residual + stage kernel.
+//
+// Elements are ordered along a Morton (Z-order) curve of their centroids and
+// cut into patches of kPatchElems consecutive elements, so a patch is a
+// compact piece of the mesh.  A node is interior to a patch when every
+// element around it lies in that patch; otherwise it is a halo node, and
+// each patch that touches it keeps one partial sum for it.
+//
+// Per patch, the fused kernel needs: its nodes (interior first, each group in
+// ascending global id), and per element the local ids of its three nodes and
+// the local slot each contribution goes to.  A node's slots are contiguous
+// and in ascending element order inside the patch (lptr), so the node sums
+// them in a fixed order; a halo node's partials are summed in ascending patch
+// order.  Everything is deterministic.
+#include <numeric>
+
+#include "cfd_internal.hpp"
+
+namespace cfd {
+
+namespace {
+inline uint64_t spread_bits(uint64_t v) {  // bit i -> bit 2i (32-bit input)
+    v &= 0xFFFFFFFFull;
+    v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+    v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+    v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+}
+}  // namespace
+
+void build_patches(PatchSet& ps, const int* inpoel, const double* X, const double* Y, int ne, int np,
+                   hipStream_t s) {
+    CFD_CHECK(ne > 0 && np > 0, "mesh: empty mesh");
+    // Morton order of element centroids over the bounding box
+    double xmin = X[0], xmax = X[0], ymin = Y[0], ymax = Y[0];
+    for (int i = 1; i < np; ++i) {
+        xmin = std::min(xmin, X[i]);
+        xmax = std::max(xmax, X[i]);
+        ymin = std::min(ymin, Y[i]);
+        ymax = std::max(ymax, Y[i]);
+    }
+    const double sx = xmax > xmin ? 2147483647.0 / (xmax - xmin) : 0.0;
+    const double sy = ymax > ymin ? 2147483647.0 / (ymax - ymin) : 0.0;
+    std::vector<std::pair<uint64_t, int>> key(ne);
+    for (int e = 0; e < ne; ++e) {
+        const int* ip = inpoel + 3 * (size_t)e;
+        double cx = 0.0, cy = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const int n = ip[k] - 1;
+            CFD_CHECK(n >= 0 && n < np, "mesh: node id out of range");
+            cx += X[n];
+            cy += Y[n];
+        }
+        cx /= 3.0;
+        cy /= 3.0;
+        const uint64_t qx = (uint64_t)((cx - xmin) * sx), qy = (uint64_t)((cy - ymin) * sy);
+        key[e] = {spread_bits(qx) | (spread_bits(qy) << 1), e};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm(ne), inv(ne);
+    for (int q = 0; q < ne; ++q) {
+        perm[q] = key[q].second;
+        inv[perm[q]] = q;
+    }
+    key.clear();
+    key.shrink_to_fit();
+
+    const int pe = kPatchElems;
+    const int npatch = (ne + pe - 1) / pe;
+    // number of distinct patches around each node
+    std::vector<int> npat(np, 0), last(np, -1);
+    for (int p = 0; p < npatch; ++p) {
+        const int q1 = std::min(ne, (p + 1) * pe);
+        for (int q = p * pe; q < q1; ++q)
+            for (int k = 0; k < 3; ++k) {
+                const int n = inpoel[3 * (size_t)perm[q] + k] - 1;
+                if (last[n] != p) {
+                    last[n] = p;
+                    ++npat[n];
+                }
+            }
+    }
+    std::vector<int> node_base(npatch + 1, 0), n_int(npatch), halo_base(npatch + 1, 0);
+    std::vector<int> pnode;
+    std::vector<uint16_t> lptr;  // nl+1 entries per patch, at node_base[p] + p
+    std::vector<uint32_t> loc(2 * (size_t)ne);
+    std::vector<int> lid(np, -1), stamp(np, -1), cnt, cursor;
+    std::vector<int> hnode_of_entry;  // global node of each halo entry
+    pnode.reserve((size_t)np * 5 / 4);
+    lptr.reserve((size_t)np * 5 / 4 + npatch);
+    int max_nodes = 0;
+    for (int p = 0; p < npatch; ++p) {
+        const int q0 = p * pe, q1 = std::min(ne, (p + 1) * pe);
+        std::vector<int> in_nodes, halo_nodes;
+        for (int q = q0; q < q1; ++q)
+            for (int k = 0; k < 3; ++k) {
+                const int n = inpoel[3 * (size_t)perm[q] + k] - 1;
+                if (stamp[n] != p) {
+                    stamp[n] = p;
+                    (npat[n] == 1 ? in_nodes : halo_nodes).push_back(n);
+                }
+            }
+        std::sort(in_nodes.begin(), in_nodes.end());
+        std::sort(halo_nodes.begin(), halo_nodes.end());
+        const int nl = (int)(in_nodes.size() + halo_nodes.size());
+        CFD_CHECK(nl <= kPatchMaxNodes, "mesh: an element patch has too many nodes");
+        max_nodes = std::max(max_nodes, nl);
+        n_int[p] = (int)in_nodes.size();
+        node_base[p + 1] = node_base[p] + nl;
+        halo_base[p + 1] = halo_base[p] + (int)halo_nodes.size();
+        int l = 0;
+        for (int n : in_nodes) { lid[n] = l++; pnode.push_back(n + 1); }
+        for (int n : halo_nodes) { lid[n] = l++; pnode.push_back(n + 1); hnode_of_entry.push_back(n); }
+        // slot layout: each node's contributions contiguous, ascending element
+        cnt.assign(nl, 0);
+        for (int q = q0; q < q1; ++q)
+            for (int k = 0; k < 3; ++k) ++cnt[lid[inpoel[3 * (size_t)perm[q] + k] - 1]];
+        cursor.assign(nl, 0);
+        int acc = 0;
+        for (int j = 0; j < nl; ++j) {
+            lptr.push_back((uint16_t)acc);
+            cursor[j] = acc;
+            acc += cnt[j];
+        }
+        lptr.push_back((uint16_t)acc);
+        for (int q = q0; q < q1; ++q) {  // 3 x 10-bit node ids, then 3 x 11-bit slots
+            uint64_t w = 0;
+            for (int k = 0; k < 3; ++k) {
+                const int j = lid[inpoel[3 * (size_t)perm[q] + k] - 1];
+                w |= (uint64_t)j << (10 * k);
+                w |= (uint64_t)cursor[j]++ << (30 + 11 * k);
+            }
+            loc[2 * (size_t)q] = (uint32_t)w;
+            loc[2 * (size_t)q + 1] = (uint32_t)(w >> 32);
+        }
+    }
+    // halo nodes: their partial entries, in ascending patch order
+    const int nentries = halo_base[npatch];
+    std::vector<int> hcount(np, 0);
+    for (int n : hnode_of_entry) ++hcount[n];
+    std::vector<int> hnode, hptr(1, 0), hpos(np, -1);
+    for (int n = 0; n < np; ++n)
+        if (hcount[n] > 0) {
+            hpos[n] = hptr.back();
+            hnode.push_back(n + 1);
+            hptr.push_back(hptr.back() + hcount[n]);
+        }
+    std::vector<int> hidx(nentries);
+    for (int t = 0; t < nentries; ++t) hidx[hpos[hnode_of_entry[t]]++] = t;  // t ascends with p
+
+    ps.npatch = npatch;
+    ps.max_nodes = max_nodes;
+    ps.nentries = nentries;
+    ps.nhalo = (int)hnode.size();
+    ps.perm.alloc(ne);
+    ps.perm.upload(perm.data(), ne, s);
+    ps.inv_perm.alloc(ne);
+    ps.inv_perm.upload(inv.data(), ne, s);
+    ps.loc.alloc(2 * (size_t)ne);
+    ps.loc.upload(loc.data(), 2 * (size_t)ne, s);
+    ps.node_base.alloc(npatch + 1);
+    ps.node_base.upload(node_base.data(), npatch + 1, s);
+    ps.n_int.alloc(npatch);
+    ps.n_int.upload(n_int.data(), npatch, s);
+    ps.halo_base.alloc(npatch + 1);
+    ps.halo_base.upload(halo_base.data(), npatch + 1, s);
+    ps.pnode.alloc(pnode.size());
+    ps.pnode.upload(pnode.data(), pnode.size(), s);
+    ps.lptr.alloc(lptr.size());
+    ps.lptr.upload(lptr.data(), lptr.size(), s);
+    ps.hnode.alloc(std::max<size_t>(1, hnode.size()));
+    ps.hnode.upload(hnode.data(), hnode.size(), s);
+    ps.hptr.alloc(hptr.size());
+    ps.hptr.upload(hptr.data(), hptr.size(), s);
+    ps.hidx.alloc(std::max(1, nentries));
+    ps.hidx.upload(hidx.data(), nentries, s);
+    ps.hpart.alloc(4 * (size_t)std::max(1, nentries));
+    CFD_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+}
+
+}  // namespace cfd

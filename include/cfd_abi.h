@@ -181,7 +181,10 @@ typedef struct {
     double pcg_tol;          /* 0 means 1e-10                                 */
     int pcg_maxit;           /* 0 means 1000                                  */
     int gcl;                 /* -1: no GCL (reference); 0/1: gcl mode          */
-    int precision;           /* 0: FP64 element kernel; 1: FP32 element       */
+    int precision;           /* 0: FP64 element kernel; 1: FP32 element
+                                arithmetic with FP64 node sums and update
+                                (fields SHOC, T_SUGN1..3 then read back the
+                                binary32 values the element kernel used)     */
 } cfd_case_desc;
 
 enum {
