@@ -69,15 +69,20 @@ __device__ inline double quality_mu(double X1, double X2, double X3, double Y1, 
     return TWOSQRT3 * ar / l;
 }
 
-// mLaplace.f90:120-148, one row per thread; column positions from the slot map
-__global__ void __launch_bounds__(kBlock)
-k_laplace(const int* __restrict__ inpoel, const int* __restrict__ esup1, const int* __restrict__ esup2,
-          const int* __restrict__ rowptr, const int* __restrict__ lslot, const double* __restrict__ X,
-          const double* __restrict__ Y, const double* __restrict__ dNx, const double* __restrict__ dNy,
-          double* __restrict__ lap, double* __restrict__ diag, int npoin) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npoin) return;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) lap[k] = 0.0;
+// mLaplace.f90:120-148, one row per thread; column positions from the slot
+// map.  A row of up to kLapRow entries accumulates in LDS (the same sequence
+// of additions per entry as in memory), column-major over the block's
+// threads so lanes hit distinct banks; longer rows accumulate in memory.
+constexpr int kLapRow = 16;
+
+// row i's contributions in esup order; acc(k) is the accumulator of CSR
+// entry r0 + k
+template <class Acc>
+__device__ inline void laplace_row(int i, int r0, const int* __restrict__ inpoel,
+                                   const int* __restrict__ esup1, const int* __restrict__ esup2,
+                                   const int* __restrict__ lslot, const double* __restrict__ X,
+                                   const double* __restrict__ Y, const double* __restrict__ dNx,
+                                   const double* __restrict__ dNy, Acc acc) {
     for (int s = esup2[i]; s < esup2[i + 1]; ++s) {
         int e = esup1[s] - 1;
         int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
@@ -86,11 +91,35 @@ k_laplace(const int* __restrict__ inpoel, const int* __restrict__ esup1, const i
         int li = (n1 == i) ? 0 : ((n2 == i) ? 1 : 2);
         double ax = dNx[3 * e + li], ay = dNy[3 * e + li];
         for (int j = 0; j < 3; ++j) {
-            int k = lslot[3 * s + j];
-            lap[k] = lap[k] + (ax * dNx[3 * e + j] + ay * dNy[3 * e + j]) * q;
+            double& a = acc(lslot[3 * s + j] - r0);
+            a = a + (ax * dNx[3 * e + j] + ay * dNy[3 * e + j]) * q;
         }
     }
-    diag[i] = lap[rowptr[i]];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_laplace(const int* __restrict__ inpoel, const int* __restrict__ esup1, const int* __restrict__ esup2,
+          const int* __restrict__ rowptr, const int* __restrict__ lslot, const double* __restrict__ X,
+          const double* __restrict__ Y, const double* __restrict__ dNx, const double* __restrict__ dNy,
+          double* __restrict__ lap, double* __restrict__ diag, int npoin) {
+    __shared__ double srow[kLapRow * kBlock];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin) return;
+    const int r0 = rowptr[i], len = rowptr[i + 1] - r0;
+    if (len <= kLapRow) {
+        double* sr = srow + threadIdx.x;
+        for (int j = 0; j < len; ++j) sr[j * kBlock] = 0.0;
+        laplace_row(i, r0, inpoel, esup1, esup2, lslot, X, Y, dNx, dNy,
+                    [sr](int k) -> double& { return sr[k * kBlock]; });
+        for (int j = 0; j < len; ++j) lap[r0 + j] = sr[j * kBlock];
+        diag[i] = sr[0];
+    } else {
+        double* row = lap + r0;
+        for (int j = 0; j < len; ++j) row[j] = 0.0;
+        laplace_row(i, r0, inpoel, esup1, esup2, lslot, X, Y, dNx, dNy,
+                    [row](int k) -> double& { return row[k]; });
+        diag[i] = row[0];
+    }
 }
 
 // gcl.f90:218-236 per node: both divergence sums in ascending element order

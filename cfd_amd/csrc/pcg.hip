@@ -39,6 +39,29 @@ enum {
     S_TOLR, S_TOLZ, S_NSCAL = 16
 };
 
+// The CSR entries of a block's rows [i0, i1) are one contiguous range
+// [rowptr[i0], rowptr[i1]).  They are staged through LDS in chunks with
+// coalesced loads; each row then visits its own entries in order,
+// f(a_ij, column j (0-based)).  Every thread of the block must call this
+// (rows without work pass an empty range b == e).
+constexpr int kSpChunk = 2048;
+
+template <class F>
+__device__ inline void staged_rows(const double* __restrict__ A, const int* __restrict__ idx,
+                                   int s0, int s1, int b, int e, double* sA, int* sI, F f) {
+    for (int c0 = s0; c0 < s1; c0 += kSpChunk) {
+        const int c1 = min(c0 + kSpChunk, s1);
+        __syncthreads();
+        for (int q = c0 + (int)threadIdx.x; q < c1; q += (int)blockDim.x) {
+            sA[q - c0] = A[q];
+            sI[q - c0] = idx[q] - 1;
+        }
+        __syncthreads();
+        const int lo = max(b, c0), hi = min(e, c1);
+        for (int j = lo; j < hi; ++j) f(sA[j - c0], sI[j - c0]);
+    }
+}
+
 // fixflag = 0; scalars reset (block 0)
 __global__ void k_fix_init(int n, int* __restrict__ fixflag, double* __restrict__ scal, double tol,
                            int maxit) {
@@ -69,18 +92,22 @@ k_residual(int n, const double* __restrict__ A, const int* __restrict__ idx,
            const double* __restrict__ diag, const double* __restrict__ x,
            const double* __restrict__ b, double* __restrict__ r, int rel,
            double* __restrict__ partials) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sA[kSpChunk];
+    __shared__ int sI[kSpChunk];
+    const int i0 = blockIdx.x * blockDim.x, i = i0 + threadIdx.x;
+    const bool act = i < n;
+    const bool row = act && !fixflag[i];
+    double dot = 0.0, dotd = 0.0;
+    staged_rows(A, idx, rowptr[i0], rowptr[min(i0 + (int)blockDim.x, n)], row ? rowptr[i] : 0,
+                row ? rowptr[i + 1] : 0, sA, sI, [&](double a, int c) {
+                    const double xc = x[c];
+                    dot = dot + a * xc;
+                    if (rel && fixflag[c]) dotd = dotd + a * xc;
+                });
     double v = 0.0, vdr = 0.0, vdz = 0.0;
-    if (i < n) {
+    if (act) {
         double ri = 0.0;
-        if (!fixflag[i]) {
-            double dot = 0.0, dotd = 0.0;
-            for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) {
-                const int c = idx[j] - 1;
-                const double xc = x[c];
-                dot = dot + A[j] * xc;
-                if (rel && fixflag[c]) dotd = dotd + A[j] * xc;
-            }
+        if (row) {
             ri = -1.0 * dot + b[i];
             if (rel) {
                 const double rd = -1.0 * dotd + b[i];
@@ -197,21 +224,25 @@ k_iter_b(int n, const double* __restrict__ A, const int* __restrict__ idx,
          double* __restrict__ y, int first, const double* __restrict__ scal,
          double* __restrict__ partials) {
     if (scal[S_DONE] != 0.0 || scal[S_STOP] != 0.0) return;
+    __shared__ double sA[kSpChunk];
+    __shared__ int sI[kSpChunk];
     const double beta = scal[S_BETA];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = blockIdx.x * blockDim.x, i = i0 + threadIdx.x;
+    const bool act = i < n;
+    const bool row = act && !fixflag[i];
+    double dot = 0.0;
+    staged_rows(A, idx, rowptr[i0], rowptr[min(i0 + (int)blockDim.x, n)], row ? rowptr[i] : 0,
+                row ? rowptr[i + 1] : 0, sA, sI, [&](double a, int c) {
+                    const double pc = first ? z[c] : beta * pold[c] + z[c];
+                    dot = dot + a * pc;
+                });
     double v = 0.0;
-    if (i < n) {
+    if (act) {
         double pi = first ? z[i] : beta * pold[i] + z[i];
         double yi;
-        if (fixflag[i]) {
+        if (!row) {
             yi = 1.0e30 * pi;
         } else {
-            double dot = 0.0;
-            for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) {
-                int c = idx[j] - 1;
-                double pc = first ? z[c] : beta * pold[c] + z[c];
-                dot = dot + A[j] * pc;
-            }
             yi = dot;
         }
         pnew[i] = pi;
