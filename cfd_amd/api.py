@@ -228,3 +228,34 @@ def biCG_ex(spMtx, spIdx, spRowptr, diagMtx, x, b, x_fix, x_fixIdx, tol=1e-10, m
                             int(fi.size), C.c_double(tol), int(maxit), int(rel), C.byref(it),
                             C.byref(rz), C.byref(cv)))
     return it.value, rz.value, bool(cv.value)
+
+
+def _f32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u16p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_ushort))
+
+
+def mlp_forward(params: np.ndarray, x_bits: np.ndarray) -> np.ndarray:
+    """Warm-start MLP on the matrix cores: params (float32, layout of
+    cfd_mlp_forward), x_bits (n, 16) BF16 bit patterns -> outputs (n, 2)."""
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x_bits, np.uint16)
+    n = x.shape[0]
+    out = np.zeros((n, 2), np.float32)
+    check(lib().cfd_mlp_forward(_f32p(p), _u16p(x), n, _f32p(out)))
+    return out
+
+
+def mlp_gradient(params: np.ndarray, x_bits: np.ndarray, t: np.ndarray):
+    """(loss, gradient) of sum_nodes sum_{o<2} (y - t)^2 / n."""
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x_bits, np.uint16)
+    tt = np.ascontiguousarray(t, np.float32)
+    n = x.shape[0]
+    g = np.zeros(p.size, np.float32)
+    loss = C.c_float()
+    check(lib().cfd_mlp_gradient(_f32p(p), _u16p(x), _f32p(tt), n, _f32p(g), C.byref(loss)))
+    return loss.value, g

@@ -626,6 +626,53 @@ int cfd_bicg_ex(int npoin, const double* spMtx, const int* spIdx, const int* spR
     })
 }
 
+int cfd_mlp_sizes(int* nparams, int* nin, int* nout) {
+    CFD_TRY({
+        if (nparams) *nparams = kMlpParams;
+        if (nin) *nin = kMlpIn;
+        if (nout) *nout = kMlpOutUsed;
+    })
+}
+
+int cfd_mlp_forward(const float* params, const unsigned short* X, int n, float* out) {
+    CFD_TRY({
+        CFD_ARG(params && X && out && n > 0, "null argument or n <= 0");
+        hipStream_t s;
+        CFD_HIP(hipStreamCreate(&s));
+        struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{s};
+        MlpWeights w;
+        w.init(params, s);
+        DBuf<unsigned short> dX(16 * (size_t)n);
+        DBuf<float> dO(2 * (size_t)n);
+        dX.upload(X, 16 * (size_t)n, s);
+        mlp_forward(w, dX.p, n, dO.p, s);
+        dO.download(out, 2 * (size_t)n, s);
+        CFD_HIP(hipStreamSynchronize(s));
+    })
+}
+
+int cfd_mlp_gradient(const float* params, const unsigned short* X, const float* T, int n, float* grad,
+                     float* loss) {
+    CFD_TRY({
+        CFD_ARG(params && X && T && grad && n > 0, "null argument or n <= 0");
+        hipStream_t s;
+        CFD_HIP(hipStreamCreate(&s));
+        struct SG { hipStream_t s; ~SG() { (void)hipStreamDestroy(s); } } sg{s};
+        MlpWeights w;
+        w.init(params, s);
+        DBuf<unsigned short> dX(16 * (size_t)n);
+        DBuf<float> dT(2 * (size_t)n);
+        dX.upload(X, 16 * (size_t)n, s);
+        dT.upload(T, 2 * (size_t)n, s);
+        mlp_gradient(w, dX.p, dT.p, n, 8, s);
+        w.grad.download(grad, kMlpParams, s);
+        float sum = 0.f;
+        w.loss.download(&sum, 1, s);
+        CFD_HIP(hipStreamSynchronize(s));
+        if (loss) *loss = sum / (float)n;
+    })
+}
+
 int cfd_gcl_put_w(cfd_mesh* m, const double* W_x, const double* W_y) {
     CFD_TRY({
         const int np = m->t.npoin;

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -251,6 +252,29 @@ void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, co
                         hipStream_t s);
 // FP32 copies of dNx/dNy in patch order
 void launch_to_f32_patch(const double* a, float* b, const int* perm, int nelem, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// MLP on the matrix cores (mlp.hip): 16 -> 64 -> 64 -> 16, tanh, tanh,
+// linear; kMlpOutUsed outputs are trained.  Parameters (FP32 master copy):
+// W1t[64][16], b1[64], W2t[64][64], b2[64], W3t[16][64], b3[16], weights
+// stored [out][in].
+// ---------------------------------------------------------------------------
+constexpr int kMlpIn = 16, kMlpHid = 64, kMlpOut = 16, kMlpOutUsed = 2;
+constexpr int kMlpW1 = 0, kMlpB1 = kMlpW1 + kMlpHid * kMlpIn, kMlpW2 = kMlpB1 + kMlpHid,
+              kMlpB2 = kMlpW2 + kMlpHid * kMlpHid, kMlpW3 = kMlpB2 + kMlpHid,
+              kMlpB3 = kMlpW3 + kMlpOut * kMlpHid, kMlpParams = kMlpB3 + kMlpOut;
+struct MlpWeights {
+    DBuf<float> w, m, v, grad, loss, gpart, lpart;  // master, Adam moments, gradient
+    DBuf<unsigned short> wb;                        // BF16 copy of the weights
+    int step = 0;
+    void init(const float* host_params, hipStream_t s);
+};
+// out[2*node + o] (FP32) for the BF16 features X[node][16]
+void mlp_forward(const MlpWeights& w, const unsigned short* X, int n, float* out, hipStream_t s);
+// gradient of sum_nodes sum_{o<2} (y - T)^2 / n into w.grad (and the loss into w.loss)
+void mlp_gradient(MlpWeights& w, const unsigned short* X, const float* T, int n,
+                  int tiles_per_block, hipStream_t s);
+void mlp_adam(MlpWeights& w, float lr, hipStream_t s);
 
 // NORMALES (subrutinas.f90:7-64) on the BC table entries that are wall nodes
 void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, const int* wnode_ptr,

@@ -147,6 +147,18 @@ int cfd_bicg_ex(int npoin, const double *spMtx, const int *spIdx, const int *spR
                 const int *x_fixIdx, int nfix, double tol, int maxit, int rel, int *iters,
                 double *rz, int *converged);
 
+/* The mesh-displacement MLP of the learned warm start (no counterpart in the
+ * reference): 16 -> 64 -> 64 -> 16 on the matrix cores, BF16 operands, FP32
+ * accumulation, tanh hidden layers; params in the layout W1t[64][16], b1[64],
+ * W2t[64][64], b2[64], W3t[16][64], b3[16] (FP32, weights [out][in]); X is
+ * n x 16 BF16 bit patterns; two outputs per node.
+ * cfd_mlp_forward: out(2,n).  cfd_mlp_gradient: grad(nparams) of
+ * sum_nodes sum_{o<2} (out - T)^2 / n and that loss. */
+int cfd_mlp_sizes(int *nparams, int *nin, int *nout);
+int cfd_mlp_forward(const float *params, const unsigned short *X, int n, float *out);
+int cfd_mlp_gradient(const float *params, const unsigned short *X, const float *T, int n,
+                     float *grad, float *loss);
+
 /* gcl_mod (gcl.f90:8-62): putW / putArea store the previous mesh velocity and
  * areas in the mesh context; main updates M in place.  mode 0 reproduces the
  * file as written (y-divergence built from W_x), mode 1 uses W_y. */
