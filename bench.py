@@ -68,6 +68,64 @@ def cpu_baseline(cs, seconds_budget: float = 20.0) -> dict:
                       f"oracle/cfd_oracle.c -O3 -fopenmp, {threads} threads"}
 
 
+def ale_compare(steps: int = 40, warmup: int = 5) -> dict:
+    """ALE step time, learned vs plain, at equal PCG tolerance: the moving
+    ellipse O-grid 512 x 1024 (524,288 nodes, 1,046,528 triangles, M=0.5,
+    Re=5000, pitching about x = -0.25).  plain = FP64 element path, each mesh
+    solve started from the previous step's solution (the reference);
+    learned = FP32 element path, mesh solves started from the learned
+    prediction (MLP on the matrix cores over the extrapolated history, with
+    fallback).  Under the relative PCG rule 1e-8 (||r|| <= 1e-8 ||r0|| of a
+    zero guess) and under the reference's absolute rule (r.z < 1e-10).
+
+    Lift/drag: pressure forces after the same steps, as the difference
+    relative to max(|Fx|, |Fy|) of the plain run.  The flow is chaotic at
+    round-off level, so the plain run is repeated from a state with a
+    one-ulp change of one node's density: that difference ("chaos floor") is
+    the least any differently-rounded run can show."""
+    import numpy as np
+    from cfd_amd.mesh import ellipse_case
+    from cfd_amd.solver import Session
+    cs = ellipse_case()
+    res = {}
+    for rule, rel, tol in (("relative_1e-8", 1, 1e-8), ("absolute_1e-10", 0, 0.0)):
+        runs = {}
+        for name, prec, ws, ulp in (("plain_fp64", 0, 0, False), ("learned_fp32", 1, 1, False),
+                                    ("plain_fp64_ulp", 0, 0, True)):
+            S = Session(cs, precision=prec, pcg_rel=rel, pcg_tol=tol, warm_start=ws)
+            if ulp:
+                U = S.get("U")
+                U[5, 0] = np.nextafter(U[5, 0], 2.0)
+                S.set("U", U)
+                S.set("U1", U)
+            S.step(warmup)
+            S.sync()
+            it0 = S.scalars()["PCG_ITSUM"]
+            t0 = time.perf_counter()
+            S.step(steps)
+            S.sync()
+            sec = time.perf_counter() - t0
+            sc = S.scalars()
+            runs[name] = {"ms_per_step": sec / steps * 1e3,
+                          "pcg_iterations_per_step": (sc["PCG_ITSUM"] - it0) / steps,
+                          "drag_fx": sc["FX1"], "lift_fy": sc["FY1"]}
+            if ws:
+                runs[name]["learned_guess_used"] = f"{sc['WS_LEARNED']:.0f}/{sc['WS_SOLVES']:.0f}"
+                runs[name]["mlp_loss"] = sc["WS_LOSS"]
+            S.close()
+        p, q, u = runs["plain_fp64"], runs["learned_fp32"], runs.pop("plain_fp64_ulp")
+        scale = max(abs(p["drag_fx"]), abs(p["lift_fy"]))
+        runs["speedup"] = p["ms_per_step"] / q["ms_per_step"]
+        runs["drag_diff"] = abs(q["drag_fx"] - p["drag_fx"]) / scale
+        runs["lift_diff"] = abs(q["lift_fy"] - p["lift_fy"]) / scale
+        runs["chaos_floor_drag"] = abs(u["drag_fx"] - p["drag_fx"]) / scale
+        runs["chaos_floor_lift"] = abs(u["lift_fy"] - p["lift_fy"]) / scale
+        res[rule] = runs
+    res["workload"] = (f"ellipse O-grid 512x1024 ({cs.npoin} nodes, {cs.nelem} triangles), M=0.5, "
+                       f"Re=5000, pitching, {warmup} untimed + {steps} timed steps")
+    return res
+
+
 def read_traffic(name: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), committed under profiles/; None when absent."""
@@ -89,6 +147,8 @@ def main():
     ap.add_argument("--nt", type=int, default=2048)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ale", action="store_true", help="skip the ALE learned-vs-plain comparison")
+    ap.add_argument("--ale-steps", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the path is a single-GPU kernel set "
@@ -163,6 +223,9 @@ def main():
         "kernels": kern,
         "state": {"dtmin": sc["DTMIN"], "time": sc["TIME"], "fx": sc["FX1"], "fy": sc["FY1"]},
     }
+    S.close()
+    if not args.no_ale:
+        out["ale"] = ale_compare(args.ale_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cs)
     print(json.dumps(out))

@@ -754,6 +754,8 @@ struct cfd_session {
     DBuf<int> ILAUX, ISET;
     BcDevice bc;
     PcgWork pcg, pcg_y;  // the x and y mesh solves
+    WarmStart ws;        // learned warm start (warm_start = 1, moving nodes)
+    bool ws_on = false;
     DBuf<double> ctl;                  // session scalars on the device (X_NCTL slots)
     double sc[CFD_NSCALARS] = {0};     // host copy, refreshed by sync_scalars()
     bool w_dirty = false;
@@ -869,13 +871,39 @@ struct cfd_session {
         tb(CFD_K_PCG);
         const double tol = d.pcg_tol > 0.0 ? d.pcg_tol : 1.0e-10;
         const int maxit = d.pcg_maxit > 0 ? d.pcg_maxit : 1000;
+        WsSolve q;
+        if (ws_on) {
+            q.XPOS = XPOS.p;
+            q.YPOS = YPOS.p;
+            q.X = X.p;
+            q.Y = Y.p;
+            q.xref = d.xref;
+            q.yref = d.yref;
+            q.dtmin = scal(CFD_S_DTMIN);
+            q.wx = &pcg;
+            q.wy = &pcg_y;
+            q.A = LAP.p;
+            q.diag = LDIAG.p;
+            q.b = BZERO.p;
+            q.fixx = POSAUX.p;
+            q.fixy = POSAUY.p;
+            q.idx = m->t.idx.p;
+            q.rowptr = m->t.rowptr.p;
+            q.fix_idx = ILAUX.p;
+            q.nfix = nn;
+            q.n_learned = scal(CFD_S_WS_LEARNED);
+            q.n_solves = scal(CFD_S_WS_SOLVES);
+            q.loss = scal(CFD_S_WS_LOSS);
+            ws.before_solves(q, s);
+        }
         const PcgSystem sys[2] = {{&pcg, XPOS.p, BZERO.p, POSAUX.p, ILAUX.p, nn},
                                   {&pcg_y, YPOS.p, BZERO.p, POSAUY.p, ILAUX.p, nn}};
         pcg_launch_multi(sys, 2, np, LAP.p, m->t.idx.p, m->t.rowptr.p, LDIAG.p, tol, maxit,
                          d.pcg_rel, s);
         te(CFD_K_PCG);
-        pcg_store_result(pcg, scal(CFD_S_PCG_ITX), scal(CFD_S_PCG_RZX), s);
-        pcg_store_result(pcg_y, scal(CFD_S_PCG_ITY), scal(CFD_S_PCG_RZY), s);
+        pcg_store_result(pcg, scal(CFD_S_PCG_ITX), scal(CFD_S_PCG_RZX), scal(CFD_S_PCG_ITSUM), s);
+        pcg_store_result(pcg_y, scal(CFD_S_PCG_ITY), scal(CFD_S_PCG_RZY), scal(CFD_S_PCG_ITSUM), s);
+        if (ws_on) ws.after_solves(q, s);
         k_move<<<grid_for(np), kBlock, 0, s>>>(X.p, X1.p, WX.p, XPOS.p, scal(CFD_S_DTMIN), np);
         k_move<<<grid_for(np), kBlock, 0, s>>>(Y.p, Y1.p, WY.p, YPOS.p, scal(CFD_S_DTMIN), np);
         CFD_HIP(hipGetLastError());
@@ -1021,6 +1049,7 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         CFD_ARG(d->nmove + d->nfix_move == 0 || d->ilaux, "ilaux");
         CFD_ARG(d->nsetedges == 0 || d->iset, "iset");
         CFD_ARG(d->precision == 0 || d->precision == 1, "precision must be 0 (FP64) or 1 (FP32 element)");
+        CFD_ARG(d->warm_start == 0 || d->warm_start == 1, "warm_start must be 0 or 1");
         auto S = std::make_unique<cfd_session>();
         S->m = m;
         S->s = m->s;
@@ -1088,6 +1117,13 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->bc.upload(h, np, &d->bc, s);
         S->pcg.ensure(np);
         S->pcg_y.ensure(np);
+        S->ws_on = d->warm_start == 1 && d->nmove > 0;
+        if (S->ws_on) {  // length scale: farthest node from the pivot
+            double l = 0.0;
+            for (int i = 0; i < np; ++i)
+                l = std::max(l, std::hypot(d->X[i] - d->xref, d->Y[i] - d->yref));
+            S->ws.init(np, l, s);
+        }
         S->sc[CFD_S_BANDERA] = 1.0;
         S->put_scalars();
         *out = S.release();

@@ -271,10 +271,39 @@ struct MlpWeights {
 };
 // out[2*node + o] (FP32) for the BF16 features X[node][16]
 void mlp_forward(const MlpWeights& w, const unsigned short* X, int n, float* out, hipStream_t s);
-// gradient of sum_nodes sum_{o<2} (y - T)^2 / n into w.grad (and the loss into w.loss)
-void mlp_gradient(MlpWeights& w, const unsigned short* X, const float* T, int n,
-                  int tiles_per_block, hipStream_t s);
+// gradient of sum_q sum_{o<2} (y - T)^2 / ns over the ns = ceil(n/stride)
+// sampled nodes q * stride into w.grad (the undivided sum into w.loss);
+// returns ns
+int mlp_gradient(MlpWeights& w, const unsigned short* X, const float* T, int n,
+                 int tiles_per_block, hipStream_t s, int stride = 1);
 void mlp_adam(MlpWeights& w, float lr, hipStream_t s);
+
+// Learned warm start of the mesh solves (warmstart.hip).  WsSolve: the two
+// solves' systems and where the session keeps its scalars.
+struct PcgWork;
+struct WsSolve {
+    double *XPOS, *YPOS;            // guesses in, solutions out
+    const double *X, *Y;            // node coordinates
+    double xref, yref;              // pivot of the body's rotation
+    const double* dtmin;            // device scalar
+    PcgWork *wx, *wy;
+    const double *A, *diag, *b, *fixx, *fixy;
+    const int *idx, *rowptr, *fix_idx;
+    int nfix;
+    double *n_learned, *n_solves, *loss;  // device scalars
+};
+struct WarmStart {
+    int np = 0, nhist = 0;
+    bool predicted = false;
+    double inv_l = 1.0;
+    MlpWeights mlp;
+    DBuf<double> h1x, h2x, h1y, h2y, ex, ey, hatx, haty, wsc, partials;
+    DBuf<unsigned short> feat;
+    DBuf<float> out, tgt;
+    void init(int npoin, double length, hipStream_t s);
+    void before_solves(const WsSolve& q, hipStream_t s);  // candidates and the choice
+    void after_solves(const WsSolve& q, hipStream_t s);   // training and history
+};
 
 // NORMALES (subrutinas.f90:7-64) on the BC table entries that are wall nodes
 void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, const int* wnode_ptr,
@@ -330,7 +359,13 @@ void pcg_launch(PcgWork& w, int npoin, const double* A, const int* idx, const in
                 const int* x_fix_idx, int nfix, double tol, int maxit, int rel, hipStream_t s);
 // Waits for the stream and returns the last solve's outcome.
 PcgResult pcg_result(const PcgWork& w, hipStream_t s);
-// Copy the last solve's iteration count and final r.z to device scalars.
-void pcg_store_result(const PcgWork& w, double* iters, double* rz, hipStream_t s);
+// Copy the last solve's iteration count and final r.z to device scalars
+// (and add the count to *itsum when given).
+void pcg_store_result(const PcgWork& w, double* iters, double* rz, double* itsum, hipStream_t s);
+// r.r of the first residual test for the guess x (fixed values applied to a
+// copy; x is not changed) into the device scalar rr_out
+void pcg_initial_rr(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
+                    const double* diag, const double* x, const double* b, const double* x_fix,
+                    const int* x_fix_idx, int nfix, double* rr_out, hipStream_t s);
 
 }  // namespace cfd

@@ -121,12 +121,13 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpDev& m, bf16x4 x4, int
 }
 
 // this lane's B operand of the input tile: features X[node][16] (BF16)
+// (sample q = node0 + l%16 is node q * stride; samples q >= n are zero)
 __device__ __forceinline__ bf16x4 load_x(const unsigned short* __restrict__ X, long long node0, int n,
-                                         int lane) {
+                                         int lane, int stride = 1) {
     const long long node = node0 + (lane & 15);
     bf16x4 r = {0, 0, 0, 0};
     if (node < n) {
-        const unsigned short* p = X + node * kMlpIn + 4 * (lane >> 4);
+        const unsigned short* p = X + node * stride * kMlpIn + 4 * (lane >> 4);
         r[0] = (short)p[0];
         r[1] = (short)p[1];
         r[2] = (short)p[2];
@@ -162,13 +163,13 @@ __device__ __forceinline__ bf16x4 transpose_tile(float* sm, const f32x4& d, int 
     return r;
 }
 
-// Training: loss sum((y - t)^2)/n_total over outputs 0..1 of the nodes
-// idx[base + q]; one wave per block, `tiles` tiles of 16 nodes per block.
+// Training: loss sum((y - t)^2)/n over outputs 0..1 of the n sampled nodes
+// q * stride (q < n); one wave per block, `tiles` tiles of 16 samples per block.
 // Writes this block's gradient partial (kMlpParams floats, parameter layout
 // of the master copy) and its loss partial.
 __global__ void __launch_bounds__(64) k_mlp_grad(MlpDev m, const unsigned short* __restrict__ X,
-                                                  const float* __restrict__ T, int n, int tiles,
-                                                  float inv_n, float* __restrict__ gpart,
+                                                  const float* __restrict__ T, int n, int stride,
+                                                  int tiles, float inv_n, float* __restrict__ gpart,
                                                   float* __restrict__ lpart) {
     __shared__ float sm[16 * 17];
     const int lane = threadIdx.x;
@@ -187,13 +188,13 @@ __global__ void __launch_bounds__(64) k_mlp_grad(MlpDev m, const unsigned short*
     for (int tt = 0; tt < tiles; ++tt) {
         const long long node0 = ((long long)blockIdx.x * tiles + tt) * 16;
         if (node0 >= n) break;  // uniform over the wave
-        const bf16x4 x4 = load_x(X, node0, n, lane);
+        const bf16x4 x4 = load_x(X, node0, n, lane, stride);
         f32x4 z1[4], h1[4], z2[4], h2[4], z3;
         mlp_tile_forward(m, x4, lane, z1, h1, z2, h2, z3);
-        // dZ3 (D layout): rows 0, 1 carry the loss; padded rows and nodes 0
-        const long long node = node0 + c;
+        // dZ3 (D layout): rows 0, 1 carry the loss; padded rows and samples 0
+        const long long q = node0 + c, node = q * stride;
         f32x4 dz3 = zero;
-        if (node < n && r4 == 0) {
+        if (q < n && r4 == 0) {
             const float d0 = z3[0] - T[2 * node], d1 = z3[1] - T[2 * node + 1];
             loss += d0 * d0 + d1 * d1;
             dz3[0] = 2.f * d0 * inv_n;
@@ -244,7 +245,7 @@ __global__ void __launch_bounds__(64) k_mlp_grad(MlpDev m, const unsigned short*
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const long long nd = node0 + r4 + j;
-            bx[j] = nd < n ? (short)X[nd * kMlpIn + c] : (short)0;
+            bx[j] = nd < n ? (short)X[nd * stride * kMlpIn + c] : (short)0;
         }
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) g3[kb] = mfma(a3, bh2[kb], g3[kb]);
@@ -369,17 +370,19 @@ void mlp_forward(const MlpWeights& w, const unsigned short* X, int n, float* out
     CFD_HIP(hipGetLastError());
 }
 
-void mlp_gradient(MlpWeights& w, const unsigned short* X, const float* T, int n, int tiles_per_block,
-                  hipStream_t s) {
-    CFD_CHECK(n > 0 && tiles_per_block > 0, "arg: MLP training needs nodes");
-    const int nb = (n + 16 * tiles_per_block - 1) / (16 * tiles_per_block);
+int mlp_gradient(MlpWeights& w, const unsigned short* X, const float* T, int n, int tiles_per_block,
+                 hipStream_t s, int stride) {
+    CFD_CHECK(n > 0 && tiles_per_block > 0 && stride > 0, "arg: MLP training needs nodes");
+    const int ns = (n + stride - 1) / stride;  // samples 0, stride, 2 stride, ... < n
+    const int nb = (ns + 16 * tiles_per_block - 1) / (16 * tiles_per_block);
     if ((size_t)nb * kMlpParams > w.gpart.n) w.gpart.alloc((size_t)nb * kMlpParams);
     if ((size_t)nb > w.lpart.n) w.lpart.alloc(nb);
-    k_mlp_grad<<<nb, 64, 0, s>>>(mlp_dev(w), X, T, n, tiles_per_block, 1.0f / (float)n, w.gpart.p,
-                                 w.lpart.p);
+    k_mlp_grad<<<nb, 64, 0, s>>>(mlp_dev(w), X, T, ns, stride, tiles_per_block, 1.0f / (float)ns,
+                                 w.gpart.p, w.lpart.p);
     k_mlp_reduce<<<grid_for(kMlpParams), kBlock, 0, s>>>(w.gpart.p, w.lpart.p, nb, w.grad.p,
                                                          w.loss.p);
     CFD_HIP(hipGetLastError());
+    return ns;
 }
 
 void mlp_adam(MlpWeights& w, float lr, hipStream_t s) {

@@ -267,15 +267,37 @@ k_fin_b(const double* __restrict__ partials, int nb, double* __restrict__ scal) 
 }
 
 __global__ void k_store_result(const double* __restrict__ scal, double* __restrict__ iters,
-                               double* __restrict__ rz) {
+                               double* __restrict__ rz, double* __restrict__ itsum) {
     *iters = scal[S_K];
     *rz = scal[S_EOLD];
+    if (itsum) *itsum = *itsum + scal[S_K];
+}
+
+__global__ void __launch_bounds__(256)
+k_fin_sum(const double* __restrict__ partials, int nb, double* __restrict__ out) {
+    const double s = combine_partials(partials, nb, 0.0, OpSum());
+    if (threadIdx.x == 0) *out = s;
 }
 
 }  // namespace
 
-void pcg_store_result(const PcgWork& w, double* iters, double* rz, hipStream_t s) {
-    k_store_result<<<1, 1, 0, s>>>(w.scal.p, iters, rz);
+void pcg_initial_rr(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
+                    const double* diag, const double* x, const double* b, const double* x_fix,
+                    const int* x_fix_idx, int nfix, double* rr_out, hipStream_t s) {
+    w.ensure(npoin);
+    const dim3 g(grid_for(npoin)), blk(kBlock);
+    CFD_HIP(hipMemcpyAsync(w.z.p, x, sizeof(double) * npoin, hipMemcpyDeviceToDevice, s));
+    k_fix_init<<<g, blk, 0, s>>>(npoin, w.fixflag.p, w.scal.p, 1.0, 0);
+    if (nfix > 0)
+        k_fix_set<<<grid_for(nfix), blk, 0, s>>>(nfix, x_fix_idx, x_fix, w.fixflag.p, w.z.p);
+    k_residual<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, diag, w.z.p, b, w.r.p, 0,
+                                 w.partial.p);
+    k_fin_sum<<<1, 256, 0, s>>>(w.partial.p, (int)g.x, rr_out);
+    CFD_HIP(hipGetLastError());
+}
+
+void pcg_store_result(const PcgWork& w, double* iters, double* rz, double* itsum, hipStream_t s) {
+    k_store_result<<<1, 1, 0, s>>>(w.scal.p, iters, rz, itsum);
     CFD_HIP(hipGetLastError());
 }
 

@@ -21,6 +21,9 @@ struct OpMin {
 struct OpSum {
     __device__ double operator()(double a, double b) const { return a + b; }
 };
+struct OpMax {
+    __device__ double operator()(double a, double b) const { return b > a ? b : a; }
+};
 
 // Block-wide reduction; the value is valid in thread 0.  Every thread of the
 // block must call it.  Block size: a multiple of 64, at most 1024.

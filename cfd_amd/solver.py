@@ -20,7 +20,9 @@ SCALAR_OF = {"DTMIN": "CFD_S_DTMIN", "TIME": "CFD_S_TIME", "DTMIN1": "CFD_S_DTMI
              "BANDERA": "CFD_S_BANDERA", "ITER": "CFD_S_ITER", "HMIN": "CFD_S_HMIN",
              "FX1": "CFD_S_FX", "FY1": "CFD_S_FY", "RM1": "CFD_S_RM", "DISN2": "CFD_S_DISN2",
              "PCG_ITX": "CFD_S_PCG_ITX", "PCG_ITY": "CFD_S_PCG_ITY", "PCG_RZX": "CFD_S_PCG_RZX",
-             "PCG_RZY": "CFD_S_PCG_RZY"}
+             "PCG_RZY": "CFD_S_PCG_RZY", "WS_LEARNED": "CFD_S_WS_LEARNED",
+             "WS_SOLVES": "CFD_S_WS_SOLVES", "WS_LOSS": "CFD_S_WS_LOSS",
+             "PCG_ITSUM": "CFD_S_PCG_ITSUM"}
 NSCALARS = SCALARS["CFD_NSCALARS"]
 WIDTH = {"U": 4, "U1": 4, "RHS": 4, "DNX": 3, "DNY": 3}
 
@@ -29,7 +31,7 @@ class Session:
     def __init__(self, cs, mesh: MeshContext | None = None, motion: int = 0,
                  motion_amp: float = 0.0, motion_omega: float = 0.0, pcg_tol: float = 0.0,
                  pcg_maxit: int = 0, gcl: int = -1, precision: int = 0, init: bool = True,
-                 pcg_rel: int = 0):
+                 pcg_rel: int = 0, warm_start: int = 0):
         self.cs = cs
         m = cs.mesh
         self.mesh = mesh if mesh is not None else MeshContext(m.inpoel, m.npoin)
@@ -55,6 +57,7 @@ class Session:
         d.motion, d.motion_amp, d.motion_omega = motion, motion_amp, motion_omega
         d.pcg_tol, d.pcg_maxit, d.gcl, d.precision = pcg_tol, pcg_maxit, gcl, precision
         d.pcg_rel = pcg_rel
+        d.warm_start = warm_start
         self._h = C.c_void_p()
         check(lib().cfd_session_create(C.byref(self._h), self.mesh.handle, C.byref(d)))
         self.npoin, self.nelem, self.nnz = m.npoin, m.nelem, self.mesh.nnz

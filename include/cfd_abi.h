@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 1
+#define CFD_ABI_VERSION 2
 
 enum {
     CFD_OK = 0,
@@ -197,6 +197,13 @@ typedef struct {
                                 arithmetic with FP64 node sums and update
                                 (fields SHOC, T_SUGN1..3 then read back the
                                 binary32 values the element kernel used)     */
+    int warm_start;          /* mesh solves with moving nodes start from
+                                0: the previous step's solution (reference)
+                                1: the learned prediction (an extrapolation of
+                                   the last two solutions plus the MLP's
+                                   correction, trained online on the solver's
+                                   own solutions) when its initial residual is
+                                   smaller, else the previous solution       */
 } cfd_case_desc;
 
 enum {
@@ -210,7 +217,12 @@ enum {
 enum {
     CFD_S_DTMIN = 0, CFD_S_TIME, CFD_S_DTMIN1, CFD_S_BANDERA, CFD_S_ITER, CFD_S_HMIN,
     CFD_S_FX, CFD_S_FY, CFD_S_RM, CFD_S_DISN2, CFD_S_PCG_ITX, CFD_S_PCG_ITY,
-    CFD_S_PCG_RZX, CFD_S_PCG_RZY, CFD_NSCALARS
+    CFD_S_PCG_RZX, CFD_S_PCG_RZY,
+    CFD_S_WS_LEARNED,  /* mesh solves started from the learned guess so far  */
+    CFD_S_WS_SOLVES,   /* mesh solves with a learned candidate so far        */
+    CFD_S_WS_LOSS,     /* the MLP's last training loss                        */
+    CFD_S_PCG_ITSUM,   /* mesh-solve iterations summed over both axes so far  */
+    CFD_NSCALARS
 };
 /* kernel timing slots (HIP events on the session stream) */
 enum {

@@ -439,3 +439,55 @@ def test_fp32_free_run_density_and_forces(cyl):
     print(f"FP32 vs FP64 after 200 steps: L2(rho) {l2:.3e}, Fx rel {dfx:.3e}")
     assert np.all(np.isfinite(b.get("U")))
     assert l2 < 1e-3 and dfx < 1e-2
+
+
+def test_learned_warm_start_cuts_mesh_solve_iterations():
+    """Moving ellipse, relative PCG rule 1e-6: the learned warm start (MLP on
+    the matrix cores over the extrapolated solution history, with fallback)
+    needs far fewer mesh-solve iterations than the previous-step guess, starts
+    from the learned guess on most solves, and the mesh agrees with the
+    reference-guess run to the solve tolerance.  The forces of this pitching
+    flow are chaotic at round-off level (tests/test_oracle_chaos.py shows it
+    on the sequential oracle): the learned run's force difference is compared
+    with the difference a one-ulp density perturbation of the plain run makes
+    over the same steps."""
+    from cfd_amd.solver import Session
+    cs = ellipse_case(48, 96)
+    nsteps = 30
+    out = {}
+    for name, ws, ulp in (("plain", 0, False), ("plain_ulp", 0, True), ("learned", 1, False)):
+        S = Session(cs, pcg_rel=1, pcg_tol=1e-6, warm_start=ws)
+        if ulp:
+            U = S.get("U")
+            U[5, 0] = np.nextafter(U[5, 0], 2.0)
+            S.set("U", U)
+            S.set("U1", U)
+        f = np.empty((nsteps, 2))
+        for k in range(nsteps):
+            S.step(1)
+            sc = S.scalars()
+            f[k] = (sc["FX1"], sc["FY1"])
+        out[name] = dict(sc=sc, X=S.get("X"), Y=S.get("Y"), f=f)
+        S.close()
+        print(f"{name}: PCG iterations {sc['PCG_ITSUM']:.0f}, learned "
+              f"{sc['WS_LEARNED']:.0f}/{sc['WS_SOLVES']:.0f}, loss {sc['WS_LOSS']:.3e}")
+    p, q = out["plain"]["sc"], out["learned"]["sc"]
+    assert q["WS_SOLVES"] == 2 * (nsteps - 2)          # candidates from the third step on
+    assert q["WS_LEARNED"] >= 0.8 * q["WS_SOLVES"]
+    assert p["PCG_ITSUM"] > 50 and q["PCG_ITSUM"] < 0.5 * p["PCG_ITSUM"]
+    assert np.isfinite(q["WS_LOSS"])
+    # the rule bounds the residual, not the error, of an ill-conditioned
+    # Laplacian: the two runs stop at different points within it
+    X0, Y0 = out["plain"]["X"], out["plain"]["Y"]
+    disp = np.abs(X0 - cs.mesh.x).max() + np.abs(Y0 - cs.mesh.y).max()
+    dx = max(np.abs(out["learned"]["X"] - X0).max(), np.abs(out["learned"]["Y"] - Y0).max())
+    print(f"mesh difference {dx:.3e} of displacement {disp:.3e}")
+    assert disp > 0 and dx <= 1e-2 * disp
+    fp = out["plain"]["f"]
+    scale = np.abs(fp).max()
+    floor = np.abs(out["plain_ulp"]["f"] - fp).max(axis=1) / scale
+    diff = np.abs(out["learned"]["f"] - fp).max(axis=1) / scale
+    print("one-ulp floor", ["%.1e" % v for v in floor[4::5]])
+    print("learned diff ", ["%.1e" % v for v in diff[4::5]])
+    assert floor[nsteps // 3:].max() > 1e-4           # 1e-4 force agreement is below round-off chaos
+    assert diff.max() < 0.1                             # the same flow
