@@ -538,6 +538,15 @@ k_stage(const int* __restrict__ esup2, const double4* __restrict__ c_calc,
 //   SUPG      tau_j (Nx_j sum_k A1 A_k + Ny_j sum_k A2 A_k)
 //   shock     3 nu (Nx_j Ux + Ny_j Uy)
 //   viscous   Nx_j sum_k K1_k + Ny_j sum_k K2_k
+// The reference's expanded Jacobian entries (calcRHS.f90:58-117) are
+// evaluated as Jacobian-vector products of the Euler fluxes at the
+// quadrature state (rho, u, v, E):
+//   dF1/dU w = (w1, 2u w1 - u^2 w0 + dp, v w1 + u w2 - uv w0, H (w1 - u w0) + u (w3 + dp))
+//   dF2/dU w = (w2, v w1 + u w2 - uv w0, 2v w2 - v^2 w0 + dp, H (w2 - v w0) + v (w3 + dp))
+//   dp = (g-1)(w3 - u w1 - v w2 + q^2/2 w0),  H = g E - (g-1) q^2/2
+// and the viscous fluxes (calcRHS.f90:119-137) through the velocity
+// gradients: K1 = (0, txx, txy, u txx + v txy + (lambda/Cv) de/dx), K2 the
+// same with y, e = E - q^2/2; both are the reference's polynomials regrouped.
 // Gradients and quadrature-point states come from FP64 node values: the
 // gradient uses differences to node 1, (U_j - U_1) N_j, so a uniform state
 // gives exactly zero despite the binary32 shape gradients.  Contributions are
@@ -561,92 +570,96 @@ __device__ inline void elem32_residual(const ElemConsts32& kc, const double4 u0,
                                        const float (&Nx)[3], const float (&Ny)[3], const float ar,
                                        const float nu, const float (&tau)[3], float4 (&out)[3]) {
 #pragma clang fp contract(fast)
-    const int p0 = 0, p1 = 1, p2 = 2;
     const double a0[4] = {u0.x, u0.y, u0.z, u0.w};
     const double a1[4] = {u1.x, u1.y, u1.z, u1.w};
     const double a2[4] = {u2.x, u2.y, u2.z, u2.w};
-    float Ux[4], Uy[4], Q[3][4];  // gradients; states at the midpoint opposite node k
+    // gradients; Q[k] = twice the state at the midpoint opposite node k
+    // (velocities and energy are ratios, so the factor 2 cancels there)
+    float Ux[4], Uy[4], Q[3][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        float d1 = (float)(a1[q] - a0[q]), d2 = (float)(a2[q] - a0[q]);
+        const float d1 = (float)(a1[q] - a0[q]), d2 = (float)(a2[q] - a0[q]);
         Ux[q] = d1 * Nx[1] + d2 * Nx[2];
         Uy[q] = d1 * Ny[1] + d2 * Ny[2];
-        Q[0][q] = (float)(0.5 * (a1[q] + a2[q]));
-        Q[1][q] = (float)(0.5 * (a0[q] + a2[q]));
-        Q[2][q] = (float)(0.5 * (a0[q] + a1[q]));
+        Q[0][q] = (float)(a1[q] + a2[q]);
+        Q[1][q] = (float)(a0[q] + a2[q]);
+        Q[2][q] = (float)(a0[q] + a1[q]);
     }
-    const float g = kc.g, gm1 = g - 1.0f, Cv = kc.Cv;
-    float mu = 0.0f, lambda = 0.0f;
+    const float g = kc.g, gm1 = g - 1.0f;
+    float mu = 0.0f, kl = 0.0f;
     if (kc.viscous) {
-        const float Tavg = (float)((T[p0] + T[p1] + T[p2]) * (1.0 / 3.0));
-        float x = Tavg / kc.T_inf;
-        float pw = x * __builtin_sqrtf(x);
+        const float Tavg = (float)((T[0] + T[1] + T[2]) * (1.0 / 3.0));
+        const float x = Tavg / kc.T_inf;
+        const float pw = x * __builtin_sqrtf(x);
         mu = kc.mu_ref * pw * (kc.T_inf + 110.0f) * rcp32(Tavg + 110.0f);
-        lambda = kc.lambda_ref * pw * (kc.T_inf + 194.0f) * rcp32(Tavg + 194.0f);
+        kl = kc.lambda_ref * pw * (kc.T_inf + 194.0f) * rcp32(Tavg + 194.0f) * rcp32(kc.Cv);
     }
+    const float mu23 = (2.0f / 3.0f) * mu;
     float A[3][4], S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0}, V1[4] = {0, 0, 0, 0},
                    V2[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float rho = Q[k][0], ir = rcp32(rho);
-        const float v1 = Q[k][1] * ir, v2 = Q[k][2] * ir, en = Q[k][3] * ir;
-        const float vsq = v1 * v1 + v2 * v2, h = vsq * gm1 - en * g;
+        const float ir2 = rcp32(Q[k][0]);  // 1 / (2 rho)
+        const float u = Q[k][1] * ir2, v = Q[k][2] * ir2, E = Q[k][3] * ir2;
+        const float uu = u * u, vv = v * v, uv = u * v, hq = 0.5f * (uu + vv);
+        const float H = g * E - gm1 * hq;
+        // A = dF1/dU Ux + dF2/dU Uy
+        const float dpx = gm1 * (Ux[3] - u * Ux[1] - v * Ux[2] + hq * Ux[0]);
+        const float dpy = gm1 * (Uy[3] - u * Uy[1] - v * Uy[2] + hq * Uy[0]);
         float* a = A[k];
         a[0] = Ux[1] + Uy[2];
-        a[1] = 0.5f * Ux[0] * (vsq * gm1 - 2.0f * v1 * v1) - Ux[1] * v1 * (g - 3.0f) -
-               Ux[2] * v2 * gm1 + Ux[3] * gm1 - Uy[0] * v1 * v2 + Uy[1] * v2 + Uy[2] * v1;
-        a[2] = -Ux[0] * v1 * v2 + Ux[1] * v2 + Ux[2] * v1 +
-               0.5f * Uy[0] * (vsq * gm1 - 2.0f * v2 * v2) - Uy[1] * v1 * gm1 -
-               Uy[2] * v2 * (g - 3.0f) + Uy[3] * gm1;
-        a[3] = Ux[0] * v1 * h - 0.5f * Ux[1] * (vsq * gm1 - 2.0f * en * g + 2.0f * v1 * v1 * gm1) -
-               Ux[2] * v1 * v2 * gm1 + Ux[3] * g * v1 + Uy[0] * v2 * h - Uy[1] * v1 * v2 * gm1 -
-               0.5f * Uy[2] * (vsq * gm1 - 2.0f * en * g + 2.0f * v2 * v2 * gm1) + Uy[3] * g * v2;
-        const float c12 = -v1 * v2 * a[0] + v1 * a[2] + v2 * a[1];
+        a[1] = 2.0f * u * Ux[1] - uu * Ux[0] + dpx + v * Uy[1] + u * Uy[2] - uv * Uy[0];
+        a[2] = v * Ux[1] + u * Ux[2] - uv * Ux[0] + 2.0f * v * Uy[2] - vv * Uy[0] + dpy;
+        a[3] = H * (Ux[1] - u * Ux[0]) + u * (Ux[3] + dpx) + H * (Uy[2] - v * Uy[0]) +
+               v * (Uy[3] + dpy);
+        // SUPG: dF1/dU a and dF2/dU a
+        const float dpa = gm1 * (a[3] - u * a[1] - v * a[2] + hq * a[0]);
+        const float c12 = v * a[1] + u * a[2] - uv * a[0], e3 = a[3] + dpa;
         S1[0] += a[1];
-        S1[1] += v1 * (3.0f - g) * a[1] - v2 * gm1 * a[2] + gm1 * a[3] +
-                 (0.5f * vsq * gm1 - v1 * v1) * a[0];
+        S1[1] += 2.0f * u * a[1] - uu * a[0] + dpa;
         S1[2] += c12;
-        S1[3] += g * v1 * a[3] - v1 * v2 * gm1 * a[2] + v1 * h * a[0] +
-                 (-0.5f * vsq * gm1 + en * g - v1 * v1 * gm1) * a[1];
+        S1[3] += H * (a[1] - u * a[0]) + u * e3;
         S2[0] += a[2];
         S2[1] += c12;
-        S2[2] += -v1 * gm1 * a[1] + v2 * (3.0f - g) * a[2] + gm1 * a[3] +
-                 (0.5f * vsq * gm1 - v2 * v2) * a[0];
-        S2[3] += g * v2 * a[3] - v1 * v2 * gm1 * a[1] + v2 * h * a[0] +
-                 (-0.5f * vsq * gm1 + en * g - v2 * v2 * gm1) * a[2];
+        S2[2] += 2.0f * v * a[2] - vv * a[0] + dpa;
+        S2[3] += H * (a[2] - v * a[0]) + v * e3;
         if (kc.viscous) {
-            const float cm = Cv * mu, ir3cv = ir * (1.0f / 3.0f) * rcp32(Cv);
-            const float sh = mu * (-Ux[0] * v2 + Ux[2] - Uy[0] * v1 + Uy[1]) * ir;
-            V1[1] += (2.0f / 3.0f) * mu * (-2.0f * Ux[0] * v1 + 2.0f * Ux[1] + Uy[0] * v2 - Uy[2]) * ir;
-            V1[2] += sh;
-            V1[3] += (cm * (-Uy[0] * v1 * v2 + 3.0f * Uy[1] * v2 - 2.0f * Uy[2] * v1) -
-                      Ux[0] * (cm * (3.0f * vsq + v1 * v1) - 3.0f * lambda * (vsq - en)) +
-                      Ux[1] * v1 * (4.0f * cm - 3.0f * lambda) + 3.0f * Ux[2] * v2 * (cm - lambda) +
-                      3.0f * Ux[3] * lambda) * ir3cv;
-            V2[1] += sh;
-            V2[2] += (2.0f / 3.0f) * mu * (Ux[0] * v1 - Ux[1] - 2.0f * Uy[0] * v2 + 2.0f * Uy[2]) * ir;
-            V2[3] += (cm * (-Ux[0] * v1 * v2 - 2.0f * Ux[1] * v2 + 3.0f * Ux[2] * v1) -
-                      Uy[0] * (cm * (3.0f * vsq + v2 * v2) - 3.0f * lambda * (vsq - en)) +
-                      3.0f * Uy[1] * v1 * (cm - lambda) + Uy[2] * v2 * (4.0f * cm - 3.0f * lambda) +
-                      3.0f * Uy[3] * lambda) * ir3cv;
+            const float ir = 2.0f * ir2;
+            const float ux = (Ux[1] - u * Ux[0]) * ir, vx = (Ux[2] - v * Ux[0]) * ir;
+            const float uy = (Uy[1] - u * Uy[0]) * ir, vy = (Uy[2] - v * Uy[0]) * ir;
+            const float txx = mu23 * (2.0f * ux - vy), tyy = mu23 * (2.0f * vy - ux);
+            const float txy = mu * (uy + vx);
+            const float ex = (Ux[3] - E * Ux[0]) * ir - (u * ux + v * vx);
+            const float ey = (Uy[3] - E * Uy[0]) * ir - (u * uy + v * vy);
+            V1[1] += txx;
+            V1[2] += txy;
+            V1[3] += u * txx + v * txy + kl * ex;
+            V2[1] += txy;
+            V2[2] += tyy;
+            V2[3] += u * txy + v * tyy + kl * ey;
         }
+    }
+    const float nu3 = 3.0f * nu, arh = 0.5f * ar;
+    float SA[4], P1[4], P2[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        SA[q] = A[0][q] + A[1][q] + A[2][q];
+        P1[q] = nu3 * Ux[q] + V1[q];
+        P2[q] = nu3 * Uy[q] + V2[q];
     }
     float r[3][4];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            float gal = 0.5f * ((A[0][q] + A[1][q] + A[2][q]) - A[j][q]);
-            float supg = tau[j] * (Nx[j] * S1[q] + Ny[j] * S2[q]);
-            float shk = 3.0f * nu * (Nx[j] * Ux[q] + Ny[j] * Uy[q]);
-            float vis = Nx[j] * V1[q] + Ny[j] * V2[q];
-            r[j][q] = ar * (gal + supg + shk + vis);
+            const float x = Nx[j] * (tau[j] * S1[q] + P1[q]) + Ny[j] * (tau[j] * S2[q] + P2[q]);
+            r[j][q] = ar * x + arh * (SA[q] - A[j][q]);
         }
     }
     if (kFuente) {
         // subrutinas.f90:1064-1078: -AR sum_k sp(j,k) (Ux wx_k + Uy wy_k)
-        const float w_x[3] = {(float)WX[p0], (float)WX[p1], (float)WX[p2]};
-        const float w_y[3] = {(float)WY[p0], (float)WY[p1], (float)WY[p2]};
+        const float w_x[3] = {(float)WX[0], (float)WX[1], (float)WX[2]};
+        const float w_y[3] = {(float)WY[0], (float)WY[1], (float)WY[2]};
         // sp(:,1) = (1/2,1/2,0), sp(:,2) = (0,1/2,1/2), sp(:,3) = (1/2,0,1/2)
         const float wx[3] = {0.5f * (w_x[0] + w_x[1]), 0.5f * (w_x[1] + w_x[2]), 0.5f * (w_x[0] + w_x[2])};
         const float wy[3] = {0.5f * (w_y[0] + w_y[1]), 0.5f * (w_y[1] + w_y[2]), 0.5f * (w_y[0] + w_y[2])};

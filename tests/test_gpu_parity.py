@@ -423,6 +423,30 @@ def test_fp32_element_step_against_fp64_oracle(cyl):
     assert got["DTMIN"] == ref["DTMIN"]
 
 
+def test_fp32_element_moving_mesh_step_against_fp64_oracle(ell):
+    """The same on the pitching ellipse: FP32 calcRHS plus the FUENTE mesh
+    velocity term, against the FP64 oracle after a teacher-forced step (the
+    state carries a nonzero mesh velocity from the oracle's own motion)."""
+    from cfd_amd.solver import Session
+    cs, o0 = ell
+    o = OracleSolver(cs)
+    o.set_state(o0.state())
+    o.disn2 = o0.disn2
+    assert np.abs(o.s["W_X"]).max() > 0
+    S = Session(cs, precision=1)
+    S.set_state(o.state())
+    S.set_scalars(DISN2=o.disn2)
+    o.step()
+    S.step(1)
+    ref, got = o.state(), S.state()
+    rel_rhs = relnorm_cols(got["RHS"], ref["RHS"])
+    print("FP32 element, moving mesh: RHS rel err per component", rel_rhs)
+    assert np.all(rel_rhs < 2e-4), rel_rhs
+    assert np.all(relnorm_cols(got["U"], ref["U"]) < 1e-7)
+    for k in ("X", "Y"):                          # FP64 mesh solve, same rule
+        assert relnorm(got[k], ref[k]) < 1e-10
+
+
 def test_fp32_free_run_density_and_forces(cyl):
     """Free-running FP32-element vs FP64 paths on the GPU: L2 density error
     and pressure force after 200 steps."""
