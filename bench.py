@@ -2,7 +2,11 @@
 
 Workload (default): viscous flow past a circle on an O-grid with Nr=1024,
 Ntheta=2048 (2,097,152 nodes, 4,190,208 triangles), M=0.8, Re=1000, fixed
-mesh -- the plain FP64 GPU path.  A step is one full time step of the
+mesh -- the FP64 GPU path (--precision fp64: element residual, node sums and
+RK stage fused per element patch; fp32: the same with binary32 element
+arithmetic; fp64ref: element kernel and node gather in the reference's
+operation order).  The other two paths are timed briefly beside it
+("variants").  A step is one full time step of the
 reference's main loop (deltat, RK with 4 calcRHS evaluations, ESTAB, the
 two mesh-Laplace PCG solves of fluidStructure, U = U1).  value = 4 * nelem *
 steps / timed seconds (every calcRHS evaluation complete, none hoisted).
@@ -11,7 +15,7 @@ The timed region starts with all inputs resident in HBM and is bracketed by
 a stream synchronisation on both sides.  Per-kernel times come from HIP
 events recorded on the session's own stream inside the timed region.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32|fp64ref]
 """
 from __future__ import annotations
 
@@ -28,22 +32,32 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "triangle-RHS evals/s, %HBM roofline; ALE step time learned vs plain at equal tol"
 
 
+PRECISION = {"fp64": 0, "fp32": 1, "fp64ref": 2}
+KERNEL = {0: "k_patch_stage<double> + k_halo_stage (FP64 residual, node sums, RK stage)",
+          1: "k_patch_stage<float> + k_halo_stage (FP32 residual, node sums, RK stage)",
+          2: "k_element_rhs (calcRHS element residual, reference order)"}
+TRAFFIC = {0: "k_patch_stage_fp64", 1: "k_patch_stage_fp32", 2: "k_element_rhs_fp64ref"}
+
+
 def element_bytes(nelem: int, npoin: int, precision: int = 0) -> int:
     """Algorithmic bytes of one launch of the timed element kernel, each array
     counted once at the width the variant reads or writes.
 
-    FP64 (k_element_rhs, calcRHS element part): per element inpoel 12 +
-    dNx,dNy 48 + area,dtl,shoc,tau1..3 48 = 108 B; per node U 32 + T 8 = 40 B
-    (theta == 0 inside RK is not read).
+    Fused paths (k_patch_stage + k_halo_stage: residual, node sums and the RK
+    stage): per element the connectivity 12 B + dNx,dNy + fac,nu,tau1..3 at
+    the element width (FP64: 48 + 40 = 88 B; FP32: 24 + 20 = 44 B); per node
+    U 32 + T 8 + M 8 + GAMM 8 = 56 B read, and written on average over the 4
+    stages (3 x T 8 + final U1 32, RHS 32, T, VX, VY, RHO, E, P, RMACH 56) / 4
+    = 36 B.
 
-    FP32 (k_patch_stage + k_halo_stage: residual, node sums and the RK stage
-    fused): per element inpoel 12 + dNx,dNy 24 + fac,nu,tau1..3 20 = 56 B;
-    per node U 32 + T 8 + M 8 + GAMM 8 = 56 B read, and written on average
-    over the 4 stages (3 x T 8 + final U1 32, RHS 32, T, VX, VY, RHO, E, P,
-    RMACH 56) / 4 = 36 B."""
+    Reference order (k_element_rhs, calcRHS element part): per element inpoel
+    12 + dNx,dNy 48 + area,dtl,shoc,tau1..3 48 = 108 B; per node U 32 + T 8 =
+    40 B (theta == 0 inside RK is not read)."""
     if precision == 0:
-        return 108 * nelem + 40 * npoin
-    return 56 * nelem + 92 * npoin
+        return 100 * nelem + 92 * npoin
+    if precision == 1:
+        return 56 * nelem + 92 * npoin
+    return 108 * nelem + 40 * npoin
 
 
 def cpu_baseline(cs, seconds_budget: float = 20.0) -> dict:
@@ -71,12 +85,13 @@ def cpu_baseline(cs, seconds_budget: float = 20.0) -> dict:
 def ale_compare(steps: int = 40, warmup: int = 5) -> dict:
     """ALE step time, learned vs plain, at equal PCG tolerance: the moving
     ellipse O-grid 512 x 1024 (524,288 nodes, 1,046,528 triangles, M=0.5,
-    Re=5000, pitching about x = -0.25).  plain = FP64 element path, each mesh
+    Re=5000, pitching about x = -0.25).  plain = fused FP64 path, each mesh
     solve started from the previous step's solution (the reference);
-    learned = FP32 element path, mesh solves started from the learned
+    learned = fused FP32-element path, mesh solves started from the learned
     prediction (MLP on the matrix cores over the extrapolated history, with
-    fallback).  Under the relative PCG rule 1e-8 (||r|| <= 1e-8 ||r0|| of a
-    zero guess) and under the reference's absolute rule (r.z < 1e-10).
+    fallback).  Under the relative PCG rules 1e-10 and 1e-8 (||r|| <= tol
+    ||r0|| of a zero guess) and under the reference's absolute rule (r.z <
+    1e-10).
 
     Lift/drag: pressure forces after the same steps, as the difference
     relative to max(|Fx|, |Fy|) of the plain run.  The flow is chaotic at
@@ -88,7 +103,8 @@ def ale_compare(steps: int = 40, warmup: int = 5) -> dict:
     from cfd_amd.solver import Session
     cs = ellipse_case()
     res = {}
-    for rule, rel, tol in (("relative_1e-8", 1, 1e-8), ("absolute_1e-10", 0, 0.0)):
+    for rule, rel, tol in (("relative_1e-10", 1, 1e-10), ("relative_1e-8", 1, 1e-8),
+                           ("absolute_1e-10", 0, 0.0)):
         runs = {}
         for name, prec, ws, ulp in (("plain_fp64", 0, 0, False), ("learned_fp32", 1, 1, False),
                                     ("plain_fp64_ulp", 0, 0, True)):
@@ -138,6 +154,42 @@ def read_traffic(name: str):
     return None
 
 
+def run_path(cs, prec: int, steps: int, warmup: int, breakdown: bool = False) -> dict:
+    """Times `steps` whole steps after `warmup` untimed ones; HIP events on the
+    session's stream time the element kernel launches inside the timed
+    region."""
+    from cfd_amd.solver import Session
+    S = Session(cs, precision=prec)
+    S.step(warmup)
+    S.sync()
+    S.timing(True, slots=["ELEMENT"])
+    S.sync()
+    t0 = time.perf_counter()
+    S.step(steps)
+    S.sync()
+    sec = time.perf_counter() - t0
+    te, ne = S.kernel_time("ELEMENT")
+    avg_elem_s = te / ne * 1e-3
+    sc = S.scalars()
+    r = {"value": 4.0 * cs.nelem * steps / sec, "ms_per_step": sec / steps * 1e3,
+         "avg_elem_us": avg_elem_s * 1e6,
+         "achieved": element_bytes(cs.nelem, cs.npoin, prec) / avg_elem_s / 1e9,
+         "state": {"dtmin": sc["DTMIN"], "time": sc["TIME"], "fx": sc["FX1"], "fy": sc["FY1"]}}
+    if breakdown:  # per-phase times: a separate, untimed pass with events on every phase
+        S.timing(True)
+        S.step(min(steps, 10))
+        S.sync()
+        kern = {}
+        for k in ("ELEMENT", "STAGE", "ESTAB", "DELTAT", "PCG", "GEOM", "STEP"):
+            t, n = S.kernel_time(k)
+            if n:
+                kern[k.lower()] = {"avg_us": round(t / n * 1e3, 2), "launches": n}
+        S.timing(False)
+        r["kernels"] = kern
+    S.close()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,7 +197,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--nr", type=int, default=1024)
     ap.add_argument("--nt", type=int, default=2048)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=list(PRECISION), default="fp64")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the short runs of the other two element paths")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ale", action="store_true", help="skip the ALE learned-vs-plain comparison")
     ap.add_argument("--ale-steps", type=int, default=40)
@@ -156,74 +210,55 @@ def main():
         sys.exit(2)
 
     from cfd_amd.mesh import cylinder_case
-    from cfd_amd.solver import Session
 
     cs = cylinder_case(args.nr, args.nt, mach=0.8, re=1000.0)
-    prec = 0 if args.precision == "fp64" else 1
-    S = Session(cs, precision=prec)
-    S.step(args.warmup)
-    S.sync()
-    # timed region: HIP events only around the element kernel launches
-    S.timing(True, slots=["ELEMENT"])
-    S.sync()
-    t0 = time.perf_counter()
-    S.step(args.steps)
-    S.sync()
-    t1 = time.perf_counter()
-    sec = t1 - t0
-    value = 4.0 * cs.nelem * args.steps / sec
-    te, ne = S.kernel_time("ELEMENT")
-    avg_elem_s = te / ne * 1e-3
-    bytes_elem = element_bytes(cs.nelem, cs.npoin, prec)
-    achieved = bytes_elem / avg_elem_s / 1e9
-    sc = S.scalars()
-    # per-phase breakdown: a separate, untimed pass with events on every phase
-    S.timing(True)
-    S.step(min(args.steps, 10))
-    S.sync()
-    kern = {}
-    for k in ("ELEMENT", "STAGE", "ESTAB", "DELTAT", "PCG", "GEOM", "STEP"):
-        t, n = S.kernel_time(k)
-        if n:
-            kern[k.lower()] = {"avg_us": round(t / n * 1e3, 2), "launches": n}
-    S.timing(False)
+    prec = PRECISION[args.precision]
+    r = run_path(cs, prec, args.steps, args.warmup, breakdown=True)
+    DTYPE = {0: "f64", 1: "f32-element/f64-accumulate", 2: "f64"}
+    PATH = {0: "fused FP64", 1: "fused FP32-element", 2: "FP64 reference-order"}
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r["value"],
         "unit": "triangle-RHS evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec / args.steps * 1e3,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == 0 else "f32-element/f64-accumulate",
+        "dtype": DTYPE[prec],
         "data": "synthetic",
         "config": {
             "workload": f"cylinder O-grid Nr={args.nr} Ntheta={args.nt} ({cs.npoin} nodes, "
-                        f"{cs.nelem} triangles), M=0.8, Re=1000, fixed mesh, "
-                        f"{'plain FP64' if prec == 0 else 'FP32-element'} path",
+                        f"{cs.nelem} triangles), M=0.8, Re=1000, fixed mesh, {PATH[prec]} path",
             "nelem": cs.nelem, "npoin": cs.npoin, "calcrhs_per_step": 4,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_element_rhs (calcRHS element residual)" if prec == 0 else
-                       "k_patch_stage + k_halo_stage (FP32 residual, node sums, RK stage)"),
-            "achieved": achieved,
+            "kernel": KERNEL[prec],
+            "achieved": r["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (read_traffic(("k_element_rhs_fp64" if prec == 0 else "k_patch_stage_fp32")
-                                     + "_cylinder1024x2048_traffic.txt")
+            "frac": r["achieved"] / HBM_PEAK_GBS,
+            "traffic": (read_traffic(TRAFFIC[prec] + "_cylinder1024x2048_traffic.txt")
                         if (args.nr, args.nt) == (1024, 2048) else None),
-            "algorithmic_bytes_per_launch": bytes_elem,
-            "avg_launch_us": avg_elem_s * 1e6,
+            "algorithmic_bytes_per_launch": element_bytes(cs.nelem, cs.npoin, prec),
+            "avg_launch_us": r["avg_elem_us"],
         },
-        "kernels": kern,
-        "state": {"dtmin": sc["DTMIN"], "time": sc["TIME"], "fx": sc["FX1"], "fy": sc["FY1"]},
+        "kernels": r["kernels"],
+        "state": r["state"],
     }
-    S.close()
+    if not args.no_variants:
+        # the other two element paths, shorter runs of the same workload
+        out["variants"] = {}
+        for name, p in PRECISION.items():
+            if p != prec:
+                v = run_path(cs, p, 20, 20)
+                out["variants"][name] = {
+                    "path": PATH[p], "value": v["value"], "ms_per_step": v["ms_per_step"],
+                    "kernel": KERNEL[p], "avg_launch_us": v["avg_elem_us"],
+                    "roofline_frac": v["achieved"] / HBM_PEAK_GBS, "steps": 20, "warmup": 20}
     if not args.no_ale:
         out["ale"] = ale_compare(args.ale_steps)
     if not args.no_cpu_baseline:

@@ -747,9 +747,10 @@ struct cfd_session {
     DBuf<double> RHS, T, VX, VY, WX, WY, X, Y, X1, Y1, RHO, E, P, RMACH, M, GAMM, XPOS, YPOS;
     DBuf<double> DTL, DT, SHOC, T1, T2, T3, AREA, DNX, DNY, HH, HHX, HHY, AREA_OLD, WXO, WYO;
     DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, POSAUY, BZERO, partials;
-    // FP32 element path: binary32 shape gradients, ESTAB outputs, slots
-    // (all in patch order)
+    // fused paths: shape gradients and ESTAB outputs in patch order, binary32
+    // (precision 1) or FP64 (precision 0)
     DBuf<float> DNX32, DNY32, NU32, TA32, TB32, TC32, FAC32;
+    DBuf<double> DNX64, DNY64, NU64, TA64, TB64, TC64, FAC64;
     PatchSet patches;
     DBuf<int> ILAUX, ISET;
     BcDevice bc;
@@ -830,32 +831,39 @@ struct cfd_session {
         bc.normales(X.p, Y.p, s);
         launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p, scal(CFD_S_HMIN),
                      partials.p, s);
-        refresh_f32_geometry();
+        refresh_patch_geometry();
         launch_masas(t, AREA.p, M.p, s);
         launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
     }
 
-    // the FP32 path's ESTAB outputs (empty on the FP64 path)
-    Elem32 elem32() {
-        Elem32 o;
-        if (d.precision != 1) return o;
-        o.nu = NU32.p;
-        o.ta = TA32.p;
-        o.tb = TB32.p;
-        o.tc = TC32.p;
-        o.fac = FAC32.p;
+    // the fused paths' ESTAB outputs (patch order)
+    template <class R>
+    ElemP<R> elem_patch(R* nu, R* ta, R* tb, R* tc, R* fac) {
+        ElemP<R> o;
+        o.nu = nu;
+        o.ta = ta;
+        o.tb = tb;
+        o.tc = tc;
+        o.fac = fac;
         o.area = AREA.p;
         o.dtl = DTL.p;
         o.perm = patches.perm.p;
         o.cte = c.cte;
         return o;
     }
+    Elem32 elem32() { return elem_patch(NU32.p, TA32.p, TB32.p, TC32.p, FAC32.p); }
+    Elem64 elem64() { return elem_patch(NU64.p, TA64.p, TB64.p, TC64.p, FAC64.p); }
+    bool fused() const { return d.precision == 0 || d.precision == 1; }
 
-    // binary32 copies of the shape gradients read by the FP32 kernel, in patch order
-    void refresh_f32_geometry() {
-        if (d.precision != 1) return;
-        launch_to_f32_patch(DNX.p, DNX32.p, patches.perm.p, ne, s);
-        launch_to_f32_patch(DNY.p, DNY32.p, patches.perm.p, ne, s);
+    // the shape gradients read by the fused kernels, in patch order
+    void refresh_patch_geometry() {
+        if (d.precision == 1) {
+            launch_to_patch(DNX.p, DNX32.p, patches.perm.p, ne, s);
+            launch_to_patch(DNY.p, DNY32.p, patches.perm.p, ne, s);
+        } else if (d.precision == 0) {
+            launch_to_patch(DNX.p, DNX64.p, patches.perm.p, ne, s);
+            launch_to_patch(DNY.p, DNY64.p, patches.perm.p, ne, s);
+        }
     }
 
     // The two biCG mesh solves of fluidStructure (meshMove.f90:91-126), then
@@ -926,25 +934,35 @@ struct cfd_session {
         const double* dtmin = scal(CFD_S_DTMIN);
 
         // RK (subrutinas.f90:645-849)
-        const bool f32 = d.precision == 1;
-        const Elem32 o32 = elem32();
-        tb(CFD_K_ESTAB);
-        launch_estab(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin, SHOC.p,
-                     T1.p, T2.p, T3.p, s, o32);
-        te(CFD_K_ESTAB);
         const bool fu = fuente_needed();
+        tb(CFD_K_ESTAB);
+        if (d.precision == 1)
+            launch_estab_patch(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
+                               elem32(), s);
+        else if (d.precision == 0)
+            launch_estab_patch(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
+                               elem64(), s);
+        else
+            launch_estab(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
+                         SHOC.p, T1.p, T2.p, T3.p, s);
+        te(CFD_K_ESTAB);
         ElemIn in{U(), nullptr, T.p, DNX.p, DNY.p, AREA.p, SHOC.p, DTL.p, T1.p, T2.p, T3.p,
                   fu ? WX.p : nullptr, fu ? WY.p : nullptr};
-        PatchIn32 in32{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DNX32.p, DNY32.p,
-                       FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
+        PatchIn<float> in32{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DNX32.p, DNY32.p,
+                            FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
+        PatchIn<double> in64{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DNX64.p, DNY64.p,
+                             FAC64.p, NU64.p, TA64.p, TB64.p, TC64.p};
         StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p, WX.p,
                       WY.p};
         const int nrk = d.nrk > 0 ? d.nrk : 4;
         for (int irk = 1; irk <= nrk; ++irk) {
             double rk_fact = 1.0 / (nrk + 1 - irk);
-            if (f32) {  // residual + gather + stage fused (timed as the element kernel)
+            if (fused()) {  // residual + gather + stage fused (timed as the element kernel)
                 tb(CFD_K_ELEMENT);
-                launch_patch_stage(t, patches, c, bc.tab, rk_fact, in32, sn, irk == nrk, s);
+                if (d.precision == 1)
+                    launch_patch_stage(t, patches, c, bc.tab, rk_fact, in32, sn, irk == nrk, s);
+                else
+                    launch_patch_stage(t, patches, c, bc.tab, rk_fact, in64, sn, irk == nrk, s);
                 te(CFD_K_ELEMENT);
                 continue;
             }
@@ -985,7 +1003,7 @@ struct cfd_session {
                 bc.normales(X.p, Y.p, s);
                 launch_deriv(t, X.p, Y.p, AREA.p, DNX.p, DNY.p, HH.p, HHX.p, HHY.p,
                              scal(CFD_S_HMIN), partials.p, s);
-                refresh_f32_geometry();
+                refresh_patch_geometry();
                 launch_gcl(t, M.p, WX.p, WY.p, WXO.p, WYO.p, DNX.p, DNY.p, AREA.p, AREA_OLD.p, dtmin,
                            d.gcl, s);
                 launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
@@ -1048,7 +1066,8 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         CFD_ARG(d->nmove >= 0 && d->nfix_move >= 0 && d->nsetedges >= 0, "list sizes");
         CFD_ARG(d->nmove + d->nfix_move == 0 || d->ilaux, "ilaux");
         CFD_ARG(d->nsetedges == 0 || d->iset, "iset");
-        CFD_ARG(d->precision == 0 || d->precision == 1, "precision must be 0 (FP64) or 1 (FP32 element)");
+        CFD_ARG(d->precision >= 0 && d->precision <= 2,
+                "precision must be 0 (FP64), 1 (FP32 element) or 2 (FP64, reference order)");
         CFD_ARG(d->warm_start == 0 || d->warm_start == 1, "warm_start must be 0 or 1");
         auto S = std::make_unique<cfd_session>();
         S->m = m;
@@ -1086,10 +1105,19 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         if (d->precision == 1) {
             for (auto* b : {&S->DNX32, &S->DNY32}) b->alloc(3 * (size_t)ne);
             for (auto* b : {&S->NU32, &S->TA32, &S->TB32, &S->TC32, &S->FAC32}) b->alloc(ne);
+        } else if (d->precision == 0) {
+            for (auto* b : {&S->DNX64, &S->DNY64}) b->alloc(3 * (size_t)ne);
+            for (auto* b : {&S->NU64, &S->TA64, &S->TB64, &S->TC64, &S->FAC64}) b->alloc(ne);
+        }
+        if (S->fused()) {
             std::vector<int> inp(3 * (size_t)ne);
             m->t.inpoel.download(inp.data(), inp.size(), s);
             CFD_HIP(hipStreamSynchronize(s));
-            build_patches(S->patches, inp.data(), d->X, d->Y, ne, np, s);
+            if (d->precision == 1)
+                build_patches(S->patches, inp.data(), d->X, d->Y, ne, np, kPatchElems, 256, s);
+            else
+                build_patches(S->patches, inp.data(), d->X, d->Y, ne, np, kPatchElems64,
+                              kPatchElems64, s);
         } else {
             S->c_calc.alloc(12 * (size_t)ne);
             S->c_fuente.alloc(12 * (size_t)ne);
@@ -1184,6 +1212,9 @@ int cfd_session_get_field(cfd_session* S, int f, double* host) {
         if (S->d.precision == 1 && estab_field)  // the binary32 values the FP32 path used
             launch_estab_widen(S->elem32(), S->patches.inv_perm.p, S->ne, S->SHOC.p, S->T1.p,
                                S->T2.p, S->T3.p, S->s);
+        if (S->d.precision == 0 && estab_field)  // from the fused FP64 path's patch order
+            launch_estab_widen(S->elem64(), S->patches.inv_perm.p, S->ne, S->SHOC.p, S->T1.p,
+                               S->T2.p, S->T3.p, S->s);
         b->download(host, n, S->s);
         CFD_HIP(hipStreamSynchronize(S->s));
     })
@@ -1201,7 +1232,7 @@ int cfd_session_set_field(cfd_session* S, int f, const double* host) {
             f == CFD_F_W_Y)
             S->move_normalized = false;
         if (f == CFD_F_DNX || f == CFD_F_DNY) {
-            S->refresh_f32_geometry();
+            S->refresh_patch_geometry();
             CFD_HIP(hipStreamSynchronize(S->s));
         }
     })

@@ -156,24 +156,35 @@ constexpr int kRedBlocks = 2048;
 int launch_deltat_partials(const Tables& t, const Consts& c, const double* area, const double* T,
                            const double* VX, const double* VY, const double* WX, const double* WY,
                            double* DT, double* partials, hipStream_t s);
-// Inputs of the FP32 element residual (nelem floats each, patch order),
-// written by ESTAB instead of its FP64 outputs when nu != nullptr: nu =
+// Inputs of the patch element residual (nelem values each, patch order, of
+// the element type R: float on the FP32 path, double on the fused FP64
+// path), written by ESTAB in place of its element-order outputs: nu =
 // SHOC*CTE, tau1..3, fac = area*dtl/3; position q holds element perm[q].
-struct Elem32 {
-    float *nu = nullptr, *ta = nullptr, *tb = nullptr, *tc = nullptr, *fac = nullptr;
+template <class R>
+struct ElemP {
+    R *nu = nullptr, *ta = nullptr, *tb = nullptr, *tc = nullptr, *fac = nullptr;
     const double *area = nullptr, *dtl = nullptr;
     const int* perm = nullptr;
     double cte = 1.0;
 };
+using Elem32 = ElemP<float>;
+using Elem64 = ElemP<double>;
 
-// ESTAB (subrutinas.f90:332-446); dtmin is a device scalar
+// ESTAB (subrutinas.f90:332-446); dtmin is a device scalar.  Element-order
+// FP64 outputs SHOC, T_SUGN1..3.
 void launch_estab(const Tables& t, const Consts& c, const double* U, const double* T,
                   const double* GAMM, const double* VX, const double* VY, const double* WX,
                   const double* WY, const double* dNx, const double* dNy, const double* dtmin,
-                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s,
-                  const Elem32& o32 = Elem32());
-// SHOC, T_SUGN1..3 (element order, FP64) from the binary32 copies of o32
-void launch_estab_widen(const Elem32& o32, const int* inv_perm, int nelem, double* shoc,
+                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s);
+// The same arithmetic, writing the patch-order inputs o of the fused kernel
+template <class R>
+void launch_estab_patch(const Tables& t, const Consts& c, const double* U, const double* T,
+                        const double* GAMM, const double* VX, const double* VY, const double* WX,
+                        const double* WY, const double* dNx, const double* dNy,
+                        const double* dtmin, const ElemP<R>& o, hipStream_t s);
+// SHOC, T_SUGN1..3 (element order, FP64) from the patch-order copies of o
+template <class R>
+void launch_estab_widen(const ElemP<R>& o, const int* inv_perm, int nelem, double* shoc,
                         double* t1, double* t2, double* t3, hipStream_t s);
 
 // Element residual: calcRHS (calcRHS.f90:36-151) and, when WX != nullptr,
@@ -217,17 +228,19 @@ void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, do
                          bool final_stage, hipStream_t s);
 
 // ---------------------------------------------------------------------------
-// Element patches for the fused FP32 path (patches.hip): Morton-ordered runs
-// of kPatchElems elements; per element (in patch order) one 64-bit word (two
-// uint32) with the local ids of its nodes (3 x 10 bits) and its local slots
-// (3 x 11 bits from bit 30); per patch its nodes,
-// interior ones first, with each node's slot range (lptr); per halo node its
-// partial-sum entries in ascending patch order.
+// Element patches for the fused paths (patches.hip): Morton-ordered runs of
+// pe elements; per element (in patch order) one 64-bit word (two uint32)
+// with the local ids of its nodes (3 x 10 bits) and its local slots (3 x 11
+// bits from bit 30); per patch its nodes, interior ones first, with each
+// node's slot range (lptr); per halo node its partial-sum entries in
+// ascending patch order.
 // ---------------------------------------------------------------------------
-constexpr int kPatchElems = 512;
+constexpr int kPatchElems = 512;     // FP32 path: float4 slots
+constexpr int kPatchElems64 = 384;   // FP64 path: double4 slots
 constexpr int kPatchMaxNodes = 700;  // keeps the fused kernel's LDS within 64 KB
 struct PatchSet {
     int npatch = 0, max_nodes = 0, nentries = 0, nhalo = 0;
+    int pe = kPatchElems, block = 256;  // elements per patch, threads per patch
     DBuf<int> perm, inv_perm;      // patch position <-> element (0-based)
     DBuf<uint32_t> loc;            // 2 per element (patch order)
     DBuf<int> node_base, n_int, halo_base;
@@ -237,21 +250,25 @@ struct PatchSet {
     DBuf<double> hpart;            // 4 per entry
 };
 void build_patches(PatchSet& ps, const int* inpoel_host, const double* X_host,
-                   const double* Y_host, int nelem, int npoin, hipStream_t s);
+                   const double* Y_host, int nelem, int npoin, int pe, int block, hipStream_t s);
 
-// Fused FP32 residual + RK stage for the patch-interior nodes, then the
+// Fused element residual + RK stage for the patch-interior nodes, then the
 // stage for the halo nodes (subrutinas.f90:679-826 with calcRHS.f90:37-141
-// and FUENTE in binary32, node sums in FP64).  Element inputs are in patch
-// order: dNx/dNy (3 floats), fac, nu, tau1..3.
-struct PatchIn32 {
+// and FUENTE; element arithmetic in R, node sums and the update in FP64).
+// Element inputs are in patch order: dNx/dNy (3 per element), fac, nu,
+// tau1..3.
+template <class R>
+struct PatchIn {
     const double *U, *T, *WX, *WY;
-    const float *dNx, *dNy, *fac, *nu, *ta, *tb, *tc;
+    const R *dNx, *dNy, *fac, *nu, *ta, *tb, *tc;
 };
+template <class R>
 void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, const BcTable& bc,
-                        double rk_fact, const PatchIn32& in, const StageNodes& nd, bool final_stage,
-                        hipStream_t s);
-// FP32 copies of dNx/dNy in patch order
-void launch_to_f32_patch(const double* a, float* b, const int* perm, int nelem, hipStream_t s);
+                        double rk_fact, const PatchIn<R>& in, const StageNodes& nd,
+                        bool final_stage, hipStream_t s);
+// dNx/dNy in patch order, rounded to R
+template <class R>
+void launch_to_patch(const double* a, R* b, const int* perm, int nelem, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // MLP on the matrix cores (mlp.hip): 16 -> 64 -> 64 -> 16, tanh, tanh,

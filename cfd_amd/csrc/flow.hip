@@ -70,11 +70,11 @@ __global__ void k_deltat_clip(double* __restrict__ DT, const double* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// ESTAB.  kF32 = false: the FP64 outputs SHOC, T_SUGN1..3 per element.
-// kF32 = true (FP32 path): thread q handles element perm[q] and writes only
-// the binary32 inputs of the fused residual, at q (patch order).
+// ESTAB.  kPatch = false: the FP64 outputs SHOC, T_SUGN1..3 per element.
+// kPatch = true (fused paths): thread q handles element perm[q] and writes
+// only the inputs of the fused residual, rounded to R, at q (patch order).
 // ---------------------------------------------------------------------------
-template <bool kF32>
+template <class R, bool kPatch>
 __global__ void __launch_bounds__(kBlock)
 k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const double* __restrict__ T,
         const double* __restrict__ GAMM, const double* __restrict__ VEL_X,
@@ -82,10 +82,10 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
         const double* __restrict__ W_Y, const double* __restrict__ dNx,
         const double* __restrict__ dNy, int nelem, double fr, double tinf, double rhoinf,
         const double* __restrict__ dtminp, double* __restrict__ shoc, double* __restrict__ t1,
-        double* __restrict__ t2, double* __restrict__ t3, Elem32 o32) {
+        double* __restrict__ t2, double* __restrict__ t3, ElemP<R> o) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nelem) return;
-    const int e = kF32 ? o32.perm[q] : q;
+    const int e = kPatch ? o.perm[q] : q;
     const double dtmin = *dtminp;
     int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
     double DNX[3] = {dNx[3 * e], dNx[3 * e + 1], dNx[3 * e + 2]};
@@ -149,12 +149,12 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
         s2 = pow_m1_2(resumen + 1.0 / (ts3 * ts3));
         s3 = pow_m1_2(resumen + 1.0 / (ts3e * ts3e));
     }
-    if (kF32) {  // inputs of the FP32 element residual, rounded once here
-        o32.nu[q] = (float)(sh * o32.cte);
-        o32.ta[q] = (float)rrr;
-        o32.tb[q] = (float)s2;
-        o32.tc[q] = (float)s3;
-        o32.fac[q] = (float)(o32.area[e] * o32.dtl[e] / 3.0);
+    if (kPatch) {  // inputs of the fused element residual, rounded once here
+        o.nu[q] = (R)(sh * o.cte);
+        o.ta[q] = (R)rrr;
+        o.tb[q] = (R)s2;
+        o.tc[q] = (R)s3;
+        o.fac[q] = (R)(o.area[e] * o.dtl[e] / 3.0);
     } else {
         shoc[e] = sh;
         t1[e] = rrr;
@@ -163,10 +163,11 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
     }
 }
 
-// SHOC, T_SUGN1..3 from the binary32 copies (FP32 path): element e's values
-// sit at patch position inv_perm[e]
-__global__ void k_estab_widen(const float* __restrict__ nu, const float* __restrict__ ta,
-                              const float* __restrict__ tb, const float* __restrict__ tc,
+// SHOC, T_SUGN1..3 from the patch-order copies (fused paths): element e's
+// values sit at patch position inv_perm[e]
+template <class R>
+__global__ void k_estab_widen(const R* __restrict__ nu, const R* __restrict__ ta,
+                              const R* __restrict__ tb, const R* __restrict__ tc,
                               const int* __restrict__ inv_perm, double inv_cte, int nelem,
                               double* __restrict__ shoc, double* __restrict__ t1,
                               double* __restrict__ t2, double* __restrict__ t3) {
@@ -531,9 +532,10 @@ k_stage(const int* __restrict__ esup2, const double4* __restrict__ c_calc,
 }
 
 // ---------------------------------------------------------------------------
-// FP32 element residual (same equations as calcRHS.f90:37-141 with theta = 0,
-// i.e. as RK calls it).  Arithmetic in binary32 with contraction; the sums
-// over the three quadrature points are taken once per element:
+// Patch element residual (same equations as calcRHS.f90:37-141 with theta =
+// 0, i.e. as RK calls it), in binary32 (FP32 path) or FP64 (fused FP64 path)
+// with contraction; the sums over the three quadrature points are taken once
+// per element:
 //   Galerkin  sum_k N(j,k) A_k         = (sum_k A_k - A_j) / 2
 //   SUPG      tau_j (Nx_j sum_k A1 A_k + Ny_j sum_k A2 A_k)
 //   shock     3 nu (Nx_j Ux + Ny_j Uy)
@@ -549,88 +551,109 @@ k_stage(const int* __restrict__ esup2, const double4* __restrict__ c_calc,
 // same with y, e = E - q^2/2; both are the reference's polynomials regrouped.
 // Gradients and quadrature-point states come from FP64 node values: the
 // gradient uses differences to node 1, (U_j - U_1) N_j, so a uniform state
-// gives exactly zero despite the binary32 shape gradients.  Contributions are
-// float4 slots summed in FP64 by the stage gather.
+// gives exactly zero whatever the rounding of the shape gradients.
+// Contributions are float4 or double4 slots summed in FP64 per node.
 // ---------------------------------------------------------------------------
-struct ElemConsts32 {
-    float g, Cv, lambda_ref, mu_ref, T_inf;
+template <class R>
+struct ElemConstsP {
+    R g, Cv, lambda_ref, mu_ref, T_inf;
     int viscous;
 };
 
-__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+template <class R> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
 
-// One element's FP32 residual at its three nodes, calcRHS plus (kFuente)
-// FUENTE, into out[j] (node j of the element).  un/T/WX/WY: the FP64 node
-// values; Nx, Ny: binary32 shape gradients; ar = area*dtl/3; nu, tau from
-// ESTAB.
-template <bool kFuente>
-__device__ inline void elem32_residual(const ElemConsts32& kc, const double4 u0, const double4 u1,
-                                       const double4 u2, const double (&T)[3],
-                                       const double (&WX)[3], const double (&WY)[3],
-                                       const float (&Nx)[3], const float (&Ny)[3], const float ar,
-                                       const float nu, const float (&tau)[3], float4 (&out)[3]) {
+__device__ __forceinline__ float4 make4(float a, float b, float c, float d) {
+    return make_float4(a, b, c, d);
+}
+__device__ __forceinline__ double4 make4(double a, double b, double c, double d) {
+    return make_double4(a, b, c, d);
+}
+
+// reciprocal: the hardware approximation in binary32, a rounded division in FP64
+__device__ __forceinline__ float rcpR(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcpR(double x) { return 1.0 / x; }
+
+// One element's residual at its three nodes, calcRHS plus (kFuente) FUENTE,
+// into out[j] (node j of the element), in the arithmetic of R.  u0..u2,
+// T, WX, WY: the FP64 node values; Nx, Ny: shape gradients; ar =
+// area*dtl/3; nu, tau from ESTAB.
+template <class R, bool kFuente>
+__device__ inline void elem_residual(const ElemConstsP<R>& kc, const double4 u0, const double4 u1,
+                                     const double4 u2, const double (&T)[3],
+                                     const double (&WX)[3], const double (&WY)[3],
+                                     const R (&Nx)[3], const R (&Ny)[3], const R ar, const R nu,
+                                     const R (&tau)[3], typename Vec4<R>::type (&out)[3]) {
 #pragma clang fp contract(fast)
     const double a0[4] = {u0.x, u0.y, u0.z, u0.w};
     const double a1[4] = {u1.x, u1.y, u1.z, u1.w};
     const double a2[4] = {u2.x, u2.y, u2.z, u2.w};
     // gradients; Q[k] = twice the state at the midpoint opposite node k
     // (velocities and energy are ratios, so the factor 2 cancels there)
-    float Ux[4], Uy[4], Q[3][4];
+    R Ux[4], Uy[4], Q[3][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const float d1 = (float)(a1[q] - a0[q]), d2 = (float)(a2[q] - a0[q]);
+        const R d1 = (R)(a1[q] - a0[q]), d2 = (R)(a2[q] - a0[q]);
         Ux[q] = d1 * Nx[1] + d2 * Nx[2];
         Uy[q] = d1 * Ny[1] + d2 * Ny[2];
-        Q[0][q] = (float)(a1[q] + a2[q]);
-        Q[1][q] = (float)(a0[q] + a2[q]);
-        Q[2][q] = (float)(a0[q] + a1[q]);
+        Q[0][q] = (R)(a1[q] + a2[q]);
+        Q[1][q] = (R)(a0[q] + a2[q]);
+        Q[2][q] = (R)(a0[q] + a1[q]);
     }
-    const float g = kc.g, gm1 = g - 1.0f;
-    float mu = 0.0f, kl = 0.0f;
-    if (kc.viscous) {
-        const float Tavg = (float)((T[0] + T[1] + T[2]) * (1.0 / 3.0));
-        const float x = Tavg / kc.T_inf;
-        const float pw = x * __builtin_sqrtf(x);
-        mu = kc.mu_ref * pw * (kc.T_inf + 110.0f) * rcp32(Tavg + 110.0f);
-        kl = kc.lambda_ref * pw * (kc.T_inf + 194.0f) * rcp32(Tavg + 194.0f) * rcp32(kc.Cv);
+    const R g = kc.g, gm1 = g - (R)1;
+    R mu = 0, kl = 0;
+    if (kc.viscous) {  // Sutherland laws at the element-average temperature
+        if constexpr (sizeof(R) == 8) {
+            const double Tavg = (T[0] + T[1] + T[2]) / 3.0;
+            const double pw = pow_3_2(Tavg / kc.T_inf);
+            mu = kc.mu_ref * pw * (kc.T_inf + 110.0) / (Tavg + 110.0);
+            kl = kc.lambda_ref * pw * (kc.T_inf + 194.0) / (Tavg + 194.0) / kc.Cv;
+        } else {
+            const float Tavg = (float)((T[0] + T[1] + T[2]) * (1.0 / 3.0));
+            const float x = Tavg / kc.T_inf;
+            const float pw = x * __builtin_sqrtf(x);
+            mu = kc.mu_ref * pw * (kc.T_inf + 110.0f) * rcpR(Tavg + 110.0f);
+            kl = kc.lambda_ref * pw * (kc.T_inf + 194.0f) * rcpR(Tavg + 194.0f) * rcpR(kc.Cv);
+        }
     }
-    const float mu23 = (2.0f / 3.0f) * mu;
-    float A[3][4], S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0}, V1[4] = {0, 0, 0, 0},
-                   V2[4] = {0, 0, 0, 0};
+    const R mu23 = (R)(2.0 / 3.0) * mu;
+    R A[3][4], S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0}, V1[4] = {0, 0, 0, 0},
+               V2[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float ir2 = rcp32(Q[k][0]);  // 1 / (2 rho)
-        const float u = Q[k][1] * ir2, v = Q[k][2] * ir2, E = Q[k][3] * ir2;
-        const float uu = u * u, vv = v * v, uv = u * v, hq = 0.5f * (uu + vv);
-        const float H = g * E - gm1 * hq;
+        const R ir2 = rcpR(Q[k][0]);  // 1 / (2 rho)
+        const R u = Q[k][1] * ir2, v = Q[k][2] * ir2, E = Q[k][3] * ir2;
+        const R uu = u * u, vv = v * v, uv = u * v, hq = (R)0.5 * (uu + vv);
+        const R H = g * E - gm1 * hq;
         // A = dF1/dU Ux + dF2/dU Uy
-        const float dpx = gm1 * (Ux[3] - u * Ux[1] - v * Ux[2] + hq * Ux[0]);
-        const float dpy = gm1 * (Uy[3] - u * Uy[1] - v * Uy[2] + hq * Uy[0]);
-        float* a = A[k];
+        const R dpx = gm1 * (Ux[3] - u * Ux[1] - v * Ux[2] + hq * Ux[0]);
+        const R dpy = gm1 * (Uy[3] - u * Uy[1] - v * Uy[2] + hq * Uy[0]);
+        R* a = A[k];
         a[0] = Ux[1] + Uy[2];
-        a[1] = 2.0f * u * Ux[1] - uu * Ux[0] + dpx + v * Uy[1] + u * Uy[2] - uv * Uy[0];
-        a[2] = v * Ux[1] + u * Ux[2] - uv * Ux[0] + 2.0f * v * Uy[2] - vv * Uy[0] + dpy;
+        a[1] = (R)2 * u * Ux[1] - uu * Ux[0] + dpx + v * Uy[1] + u * Uy[2] - uv * Uy[0];
+        a[2] = v * Ux[1] + u * Ux[2] - uv * Ux[0] + (R)2 * v * Uy[2] - vv * Uy[0] + dpy;
         a[3] = H * (Ux[1] - u * Ux[0]) + u * (Ux[3] + dpx) + H * (Uy[2] - v * Uy[0]) +
                v * (Uy[3] + dpy);
         // SUPG: dF1/dU a and dF2/dU a
-        const float dpa = gm1 * (a[3] - u * a[1] - v * a[2] + hq * a[0]);
-        const float c12 = v * a[1] + u * a[2] - uv * a[0], e3 = a[3] + dpa;
+        const R dpa = gm1 * (a[3] - u * a[1] - v * a[2] + hq * a[0]);
+        const R c12 = v * a[1] + u * a[2] - uv * a[0], e3 = a[3] + dpa;
         S1[0] += a[1];
-        S1[1] += 2.0f * u * a[1] - uu * a[0] + dpa;
+        S1[1] += (R)2 * u * a[1] - uu * a[0] + dpa;
         S1[2] += c12;
         S1[3] += H * (a[1] - u * a[0]) + u * e3;
         S2[0] += a[2];
         S2[1] += c12;
-        S2[2] += 2.0f * v * a[2] - vv * a[0] + dpa;
+        S2[2] += (R)2 * v * a[2] - vv * a[0] + dpa;
         S2[3] += H * (a[2] - v * a[0]) + v * e3;
         if (kc.viscous) {
-            const float ir = 2.0f * ir2;
-            const float ux = (Ux[1] - u * Ux[0]) * ir, vx = (Ux[2] - v * Ux[0]) * ir;
-            const float uy = (Uy[1] - u * Uy[0]) * ir, vy = (Uy[2] - v * Uy[0]) * ir;
-            const float txx = mu23 * (2.0f * ux - vy), tyy = mu23 * (2.0f * vy - ux);
-            const float txy = mu * (uy + vx);
-            const float ex = (Ux[3] - E * Ux[0]) * ir - (u * ux + v * vx);
-            const float ey = (Uy[3] - E * Uy[0]) * ir - (u * uy + v * vy);
+            const R ir = (R)2 * ir2;
+            const R ux = (Ux[1] - u * Ux[0]) * ir, vx = (Ux[2] - v * Ux[0]) * ir;
+            const R uy = (Uy[1] - u * Uy[0]) * ir, vy = (Uy[2] - v * Uy[0]) * ir;
+            const R txx = mu23 * ((R)2 * ux - vy), tyy = mu23 * ((R)2 * vy - ux);
+            const R txy = mu * (uy + vx);
+            const R ex = (Ux[3] - E * Ux[0]) * ir - (u * ux + v * vx);
+            const R ey = (Uy[3] - E * Uy[0]) * ir - (u * uy + v * vy);
             V1[1] += txx;
             V1[2] += txy;
             V1[3] += u * txx + v * txy + kl * ex;
@@ -639,36 +662,37 @@ __device__ inline void elem32_residual(const ElemConsts32& kc, const double4 u0,
             V2[3] += u * txy + v * tyy + kl * ey;
         }
     }
-    const float nu3 = 3.0f * nu, arh = 0.5f * ar;
-    float SA[4], P1[4], P2[4];
+    const R nu3 = (R)3 * nu, arh = (R)0.5 * ar;
+    R SA[4], P1[4], P2[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         SA[q] = A[0][q] + A[1][q] + A[2][q];
         P1[q] = nu3 * Ux[q] + V1[q];
         P2[q] = nu3 * Uy[q] + V2[q];
     }
-    float r[3][4];
+    R r[3][4];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float x = Nx[j] * (tau[j] * S1[q] + P1[q]) + Ny[j] * (tau[j] * S2[q] + P2[q]);
+            const R x = Nx[j] * (tau[j] * S1[q] + P1[q]) + Ny[j] * (tau[j] * S2[q] + P2[q]);
             r[j][q] = ar * x + arh * (SA[q] - A[j][q]);
         }
     }
     if (kFuente) {
         // subrutinas.f90:1064-1078: -AR sum_k sp(j,k) (Ux wx_k + Uy wy_k)
-        const float w_x[3] = {(float)WX[0], (float)WX[1], (float)WX[2]};
-        const float w_y[3] = {(float)WY[0], (float)WY[1], (float)WY[2]};
+        const R w_x[3] = {(R)WX[0], (R)WX[1], (R)WX[2]};
+        const R w_y[3] = {(R)WY[0], (R)WY[1], (R)WY[2]};
         // sp(:,1) = (1/2,1/2,0), sp(:,2) = (0,1/2,1/2), sp(:,3) = (1/2,0,1/2)
-        const float wx[3] = {0.5f * (w_x[0] + w_x[1]), 0.5f * (w_x[1] + w_x[2]), 0.5f * (w_x[0] + w_x[2])};
-        const float wy[3] = {0.5f * (w_y[0] + w_y[1]), 0.5f * (w_y[1] + w_y[2]), 0.5f * (w_y[0] + w_y[2])};
-        const float sp[3][3] = {{0.5f, 0.0f, 0.5f}, {0.5f, 0.5f, 0.0f}, {0.0f, 0.5f, 0.5f}};
+        const R h = (R)0.5;
+        const R wx[3] = {h * (w_x[0] + w_x[1]), h * (w_x[1] + w_x[2]), h * (w_x[0] + w_x[2])};
+        const R wy[3] = {h * (w_y[0] + w_y[1]), h * (w_y[1] + w_y[2]), h * (w_y[0] + w_y[2])};
+        const R sp[3][3] = {{h, 0, h}, {h, h, 0}, {0, h, h}};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                float acc = 0.0f;
+                R acc = 0;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) acc += sp[j][k] * (Ux[q] * wx[k] + Uy[q] * wy[k]);
                 r[j][q] += -ar * acc;
@@ -676,34 +700,36 @@ __device__ inline void elem32_residual(const ElemConsts32& kc, const double4 u0,
         }
     }
 #pragma unroll
-    for (int j = 0; j < 3; ++j) out[j] = make_float4(r[j][0], r[j][1], r[j][2], r[j][3]);
+    for (int j = 0; j < 3; ++j) out[j] = make4(r[j][0], r[j][1], r[j][2], r[j][3]);
 }
 
-// Fused FP32 residual + stage, one patch per block: node values to LDS; each
-// thread computes elements of the patch into LDS slots; each node sums its
-// slots in FP64, then interior nodes take the RK stage and halo nodes store
-// their partial sum.
+// Fused residual + stage, one patch per block: node values to LDS; each
+// thread computes elements of the patch into LDS slots (float4 or double4);
+// each node sums its slots in FP64, then interior nodes take the RK stage and
+// halo nodes store their partial sum.
 struct PatchDev {
     const uint32_t* loc;
     const int *node_base, *n_int, *halo_base, *pnode;
     const uint16_t* lptr;
     double* hpart;
-    int nelem, maxn;
+    int nelem, maxn, pe;
 };
 
-template <bool kFuente, bool kFinal>
-__global__ void __launch_bounds__(kBlock)
-k_patch_stage(PatchDev pd, ElemConsts32 kc, PatchIn32 in, StageDev a) {
+template <class R, bool kFuente, bool kFinal>
+__global__ void __launch_bounds__(512)
+k_patch_stage(PatchDev pd, ElemConstsP<R> kc, PatchIn<R> in, StageDev a) {
+    using V4 = typename Vec4<R>::type;
     extern __shared__ double4 smem4[];
-    const int maxn = pd.maxn;
-    double4* sU = smem4;                                                // maxn
-    double2* sW = reinterpret_cast<double2*>(sU + maxn);                // maxn (kFuente)
-    float4* sC = reinterpret_cast<float4*>(sW + (kFuente ? maxn : 0));  // 3*kPatchElems
-    double* sT = reinterpret_cast<double*>(sC + 3 * kPatchElems);       // maxn
-    uint16_t* sP = reinterpret_cast<uint16_t*>(sT + maxn);              // maxn+1
+    const int maxn = pd.maxn, pe = pd.pe;
+    double4* sU = smem4;                                          // maxn
+    double2* sW = reinterpret_cast<double2*>(sU + maxn);          // maxn (kFuente)
+    V4* sC = reinterpret_cast<V4*>(sW + (kFuente ? maxn : 0));    // 3*pe
+    double* sT = reinterpret_cast<double*>(sC + 3 * pe);          // maxn
+    uint16_t* sP = reinterpret_cast<uint16_t*>(sT + maxn);        // maxn+1
     const int p = blockIdx.x;
     const int nb = pd.node_base[p], nl = pd.node_base[p + 1] - nb, nint = pd.n_int[p];
-    const int e0 = p * kPatchElems, nep = min(kPatchElems, pd.nelem - e0);
+    const long long e0 = (long long)p * pe;
+    const int nep = (int)min((long long)pe, pd.nelem - e0);
     for (int l = threadIdx.x; l < nl; l += blockDim.x) {
         const int g = pd.pnode[nb + l] - 1;
         sU[l] = ld4(in.U + 4 * (long long)g);
@@ -718,9 +744,9 @@ k_patch_stage(PatchDev pd, ElemConsts32 kc, PatchIn32 in, StageDev a) {
         const int ln[3] = {(int)(w & 1023u), (int)((w >> 10) & 1023u), (int)((w >> 20) & 1023u)};
         const int ls[3] = {(int)((w >> 30) & 2047u), (int)((w >> 41) & 2047u),
                            (int)((w >> 52) & 2047u)};
-        const float Nx[3] = {in.dNx[3 * e], in.dNx[3 * e + 1], in.dNx[3 * e + 2]};
-        const float Ny[3] = {in.dNy[3 * e], in.dNy[3 * e + 1], in.dNy[3 * e + 2]};
-        const float tau[3] = {in.ta[e], in.tb[e], in.tc[e]};
+        const R Nx[3] = {in.dNx[3 * e], in.dNx[3 * e + 1], in.dNx[3 * e + 2]};
+        const R Ny[3] = {in.dNy[3 * e], in.dNy[3 * e + 1], in.dNy[3 * e + 2]};
+        const R tau[3] = {in.ta[e], in.tb[e], in.tc[e]};
         const double T3[3] = {sT[ln[0]], sT[ln[1]], sT[ln[2]]};
         double wx3[3] = {0.0, 0.0, 0.0}, wy3[3] = {0.0, 0.0, 0.0};
         if (kFuente) {
@@ -730,9 +756,9 @@ k_patch_stage(PatchDev pd, ElemConsts32 kc, PatchIn32 in, StageDev a) {
                 wy3[k] = sW[ln[k]].y;
             }
         }
-        float4 out[3];
-        elem32_residual<kFuente>(kc, sU[ln[0]], sU[ln[1]], sU[ln[2]], T3, wx3, wy3, Nx, Ny,
-                                 in.fac[e], in.nu[e], tau, out);
+        V4 out[3];
+        elem_residual<R, kFuente>(kc, sU[ln[0]], sU[ln[1]], sU[ln[2]], T3, wx3, wy3, Nx, Ny,
+                                  in.fac[e], in.nu[e], tau, out);
 #pragma unroll
         for (int k = 0; k < 3; ++k) sC[ls[k]] = out[k];
     }
@@ -740,7 +766,7 @@ k_patch_stage(PatchDev pd, ElemConsts32 kc, PatchIn32 in, StageDev a) {
     for (int l = threadIdx.x; l < nl; l += blockDim.x) {
         double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
         for (int s = sP[l]; s < sP[l + 1]; ++s) {
-            const float4 c = sC[s];
+            const V4 c = sC[s];
             r0 = r0 + (double)c.x;
             r1 = r1 + (double)c.y;
             r2 = r2 + (double)c.z;
@@ -773,14 +799,15 @@ k_halo_stage(const int* __restrict__ hnode, const int* __restrict__ hptr,
     stage_node<kFinal>(a, g, ld4(U + 4 * (long long)g), r0, r1, r2, r3);
 }
 
-// b[3q+k] = (float)a[3 perm[q] + k]
-__global__ void k_to_f32_patch(const double* __restrict__ a, float* __restrict__ b,
-                               const int* __restrict__ perm, int nelem) {
+// b[3q+k] = (R)a[3 perm[q] + k]
+template <class R>
+__global__ void k_to_patch(const double* __restrict__ a, R* __restrict__ b,
+                           const int* __restrict__ perm, int nelem) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nelem) return;
     const long long e = perm[q];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) b[3 * (long long)q + k] = (float)a[3 * e + k];
+    for (int k = 0; k < 3; ++k) b[3 * (long long)q + k] = (R)a[3 * e + k];
 }
 
 }  // namespace
@@ -808,25 +835,47 @@ void launch_deltat(const Tables& t, const Consts& c, const double* area, const d
 void launch_estab(const Tables& t, const Consts& c, const double* U, const double* T,
                   const double* GAMM, const double* VX, const double* VY, const double* WX,
                   const double* WY, const double* dNx, const double* dNy, const double* dtmin,
-                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s,
-                  const Elem32& o32) {
-    if (o32.nu)
-        k_estab<true><<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY,
-                                                           dNx, dNy, t.nelem, c.fr, c.t_inf,
-                                                           c.rho_inf, dtmin, shoc, t1, t2, t3, o32);
-    else
-        k_estab<false><<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY,
-                                                            dNx, dNy, t.nelem, c.fr, c.t_inf,
-                                                            c.rho_inf, dtmin, shoc, t1, t2, t3, o32);
+                  double* shoc, double* t1, double* t2, double* t3, hipStream_t s) {
+    k_estab<double, false><<<grid_for(t.nelem), kBlock, 0, s>>>(
+        t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy, t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin,
+        shoc, t1, t2, t3, Elem64());
     CFD_HIP(hipGetLastError());
 }
 
-void launch_estab_widen(const Elem32& o32, const int* inv_perm, int nelem, double* shoc,
-                        double* t1, double* t2, double* t3, hipStream_t s) {
-    k_estab_widen<<<grid_for(nelem), kBlock, 0, s>>>(o32.nu, o32.ta, o32.tb, o32.tc, inv_perm,
-                                                     1.0 / o32.cte, nelem, shoc, t1, t2, t3);
+template <class R>
+void launch_estab_patch(const Tables& t, const Consts& c, const double* U, const double* T,
+                        const double* GAMM, const double* VX, const double* VY, const double* WX,
+                        const double* WY, const double* dNx, const double* dNy,
+                        const double* dtmin, const ElemP<R>& o, hipStream_t s) {
+    CFD_CHECK(o.nu && o.perm, "internal: patch outputs missing");
+    k_estab<R, true><<<grid_for(t.nelem), kBlock, 0, s>>>(
+        t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy, t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin,
+        nullptr, nullptr, nullptr, nullptr, o);
     CFD_HIP(hipGetLastError());
 }
+
+template <class R>
+void launch_estab_widen(const ElemP<R>& o, const int* inv_perm, int nelem, double* shoc,
+                        double* t1, double* t2, double* t3, hipStream_t s) {
+    k_estab_widen<R><<<grid_for(nelem), kBlock, 0, s>>>(o.nu, o.ta, o.tb, o.tc, inv_perm,
+                                                        1.0 / o.cte, nelem, shoc, t1, t2, t3);
+    CFD_HIP(hipGetLastError());
+}
+
+template void launch_estab_patch<float>(const Tables&, const Consts&, const double*, const double*,
+                                         const double*, const double*, const double*,
+                                         const double*, const double*, const double*,
+                                         const double*, const double*, const ElemP<float>&,
+                                         hipStream_t);
+template void launch_estab_patch<double>(const Tables&, const Consts&, const double*,
+                                          const double*, const double*, const double*,
+                                          const double*, const double*, const double*,
+                                          const double*, const double*, const double*,
+                                          const ElemP<double>&, hipStream_t);
+template void launch_estab_widen<float>(const ElemP<float>&, const int*, int, double*, double*,
+                                        double*, double*, hipStream_t);
+template void launch_estab_widen<double>(const ElemP<double>&, const int*, int, double*, double*,
+                                         double*, double*, hipStream_t);
 
 static ElemConsts elem_consts(const Consts& c) {
     ElemConsts k;
@@ -915,30 +964,31 @@ void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, do
     CFD_HIP(hipGetLastError());
 }
 
-static size_t patch_lds_bytes(int maxn, bool fuente) {
+static size_t patch_lds_bytes(int maxn, bool fuente, int pe, size_t slot) {
     return (size_t)maxn * (sizeof(double4) + sizeof(double) + (fuente ? sizeof(double2) : 0)) +
-           3 * (size_t)kPatchElems * sizeof(float4) + (size_t)(maxn + 1) * sizeof(uint16_t);
+           3 * (size_t)pe * slot + (size_t)(maxn + 1) * sizeof(uint16_t);
 }
 
-template <bool kFuente, bool kFinal>
-static void patch_launch_t(const PatchSet& ps, const PatchDev& pd, const ElemConsts32& kc,
-                           const PatchIn32& in, const StageDev& a, hipStream_t s) {
-    const size_t lds = patch_lds_bytes(pd.maxn, kFuente);
-    k_patch_stage<kFuente, kFinal><<<ps.npatch, kBlock, lds, s>>>(pd, kc, in, a);
+template <class R, bool kFuente, bool kFinal>
+static void patch_launch_t(const PatchSet& ps, const PatchDev& pd, const ElemConstsP<R>& kc,
+                           const PatchIn<R>& in, const StageDev& a, hipStream_t s) {
+    const size_t lds = patch_lds_bytes(pd.maxn, kFuente, pd.pe, sizeof(typename Vec4<R>::type));
+    k_patch_stage<R, kFuente, kFinal><<<ps.npatch, ps.block, lds, s>>>(pd, kc, in, a);
     if (ps.nhalo > 0)
         k_halo_stage<kFinal><<<grid_for(ps.nhalo), kBlock, 0, s>>>(
             ps.hnode.p, ps.hptr.p, ps.hidx.p, ps.hpart.p, in.U, a, ps.nhalo);
 }
 
+template <class R>
 void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, const BcTable& bc,
-                        double rk_fact, const PatchIn32& in, const StageNodes& nd, bool final_stage,
-                        hipStream_t s) {
-    ElemConsts32 kc;
-    kc.g = (float)c.gama;
-    kc.Cv = (float)c.fcv;
-    kc.lambda_ref = (float)c.fk;
-    kc.mu_ref = (float)c.fmu;
-    kc.T_inf = (float)c.t_inf;
+                        double rk_fact, const PatchIn<R>& in, const StageNodes& nd,
+                        bool final_stage, hipStream_t s) {
+    ElemConstsP<R> kc;
+    kc.g = (R)c.gama;
+    kc.Cv = (R)c.fcv;
+    kc.lambda_ref = (R)c.fk;
+    kc.mu_ref = (R)c.fmu;
+    kc.T_inf = (R)c.t_inf;
     kc.viscous = c.fmu > 2.2250738585072014e-308 ? 1 : 0;
     PatchDev pd;
     pd.loc = ps.loc.p;
@@ -950,20 +1000,33 @@ void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, co
     pd.hpart = ps.hpart.p;
     pd.nelem = t.nelem;
     pd.maxn = ps.max_nodes;
-    CFD_CHECK(patch_lds_bytes(pd.maxn, true) <= 65536, "mesh: element patch exceeds the LDS budget");
+    pd.pe = ps.pe;
+    CFD_CHECK(ps.block >= 64 && ps.block <= 512 && ps.block % 64 == 0, "internal: patch block size");
+    CFD_CHECK(patch_lds_bytes(pd.maxn, true, ps.pe, sizeof(typename Vec4<R>::type)) <= 65536,
+              "mesh: element patch exceeds the LDS budget");
     const StageDev a = stage_dev(c, bc, rk_fact, nd);
     const bool fu = in.WX != nullptr;
-    if (fu && final_stage) patch_launch_t<true, true>(ps, pd, kc, in, a, s);
-    else if (fu) patch_launch_t<true, false>(ps, pd, kc, in, a, s);
-    else if (final_stage) patch_launch_t<false, true>(ps, pd, kc, in, a, s);
-    else patch_launch_t<false, false>(ps, pd, kc, in, a, s);
+    if (fu && final_stage) patch_launch_t<R, true, true>(ps, pd, kc, in, a, s);
+    else if (fu) patch_launch_t<R, true, false>(ps, pd, kc, in, a, s);
+    else if (final_stage) patch_launch_t<R, false, true>(ps, pd, kc, in, a, s);
+    else patch_launch_t<R, false, false>(ps, pd, kc, in, a, s);
     CFD_HIP(hipGetLastError());
 }
 
-void launch_to_f32_patch(const double* a, float* b, const int* perm, int nelem, hipStream_t s) {
+template <class R>
+void launch_to_patch(const double* a, R* b, const int* perm, int nelem, hipStream_t s) {
     if (nelem <= 0) return;
-    k_to_f32_patch<<<grid_for(nelem), kBlock, 0, s>>>(a, b, perm, nelem);
+    k_to_patch<R><<<grid_for(nelem), kBlock, 0, s>>>(a, b, perm, nelem);
     CFD_HIP(hipGetLastError());
 }
+
+template void launch_patch_stage<float>(const Tables&, const PatchSet&, const Consts&,
+                                        const BcTable&, double, const PatchIn<float>&,
+                                        const StageNodes&, bool, hipStream_t);
+template void launch_patch_stage<double>(const Tables&, const PatchSet&, const Consts&,
+                                         const BcTable&, double, const PatchIn<double>&,
+                                         const StageNodes&, bool, hipStream_t);
+template void launch_to_patch<float>(const double*, float*, const int*, int, hipStream_t);
+template void launch_to_patch<double>(const double*, double*, const int*, int, hipStream_t);
 
 }  // namespace cfd

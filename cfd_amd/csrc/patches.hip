@@ -1,7 +1,7 @@
-// Element patches for the fused FP32 residual + stage kernel.
+// Element patches for the fused residual + stage kernels (FP32 and FP64).
 //
 // Elements are ordered along a Morton (Z-order) curve of their centroids and
-// cut into patches of kPatchElems consecutive elements, so a patch is a
+// cut into patches of pe consecutive elements, so a patch is a
 // compact piece of the mesh.  A node is interior to a patch when every
 // element around it lies in that patch; otherwise it is a halo node, and
 // each patch that touches it keeps one partial sum for it.
@@ -31,8 +31,10 @@ inline uint64_t spread_bits(uint64_t v) {  // bit i -> bit 2i (32-bit input)
 }  // namespace
 
 void build_patches(PatchSet& ps, const int* inpoel, const double* X, const double* Y, int ne, int np,
-                   hipStream_t s) {
+                   int pe, int block, hipStream_t s) {
     CFD_CHECK(ne > 0 && np > 0, "mesh: empty mesh");
+    // local slots are 11-bit and local node ids 10-bit fields of the per-element word
+    CFD_CHECK(pe >= 1 && 3 * pe <= 2047 && kPatchMaxNodes <= 1023, "internal: patch size");
     // Morton order of element centroids over the bounding box
     double xmin = X[0], xmax = X[0], ymin = Y[0], ymax = Y[0];
     for (int i = 1; i < np; ++i) {
@@ -67,7 +69,6 @@ void build_patches(PatchSet& ps, const int* inpoel, const double* X, const doubl
     key.clear();
     key.shrink_to_fit();
 
-    const int pe = kPatchElems;
     const int npatch = (ne + pe - 1) / pe;
     // number of distinct patches around each node
     std::vector<int> npat(np, 0), last(np, -1);
@@ -151,6 +152,8 @@ void build_patches(PatchSet& ps, const int* inpoel, const double* X, const doubl
     for (int t = 0; t < nentries; ++t) hidx[hpos[hnode_of_entry[t]]++] = t;  // t ascends with p
 
     ps.npatch = npatch;
+    ps.pe = pe;
+    ps.block = block;
     ps.max_nodes = max_nodes;
     ps.nentries = nentries;
     ps.nhalo = (int)hnode.size();

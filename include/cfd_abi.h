@@ -193,10 +193,17 @@ typedef struct {
     double pcg_tol;          /* 0 means 1e-10                                 */
     int pcg_maxit;           /* 0 means 1000                                  */
     int gcl;                 /* -1: no GCL (reference); 0/1: gcl mode          */
-    int precision;           /* 0: FP64 element kernel; 1: FP32 element
-                                arithmetic with FP64 node sums and update
-                                (fields SHOC, T_SUGN1..3 then read back the
-                                binary32 values the element kernel used)     */
+    int precision;           /* 0: FP64, element residual, node sums and
+                                stage fused per element patch (the residual
+                                regrouped, sums per patch: within 1e-12 of
+                                the reference order per step);
+                                1: FP32 element arithmetic with FP64 node
+                                sums and update, fused the same way;
+                                2: FP64 in the reference's operation and
+                                accumulation order (element kernel, then the
+                                node gather in element order).
+                                With 0 and 1, fields SHOC, T_SUGN1..3 read
+                                back the values the fused kernel used       */
     int warm_start;          /* mesh solves with moving nodes start from
                                 0: the previous step's solution (reference)
                                 1: the learned prediction (an extrapolation of

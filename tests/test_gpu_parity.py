@@ -329,22 +329,26 @@ def test_gcl(ell, mode):
     assert bitdiff(M, ref) == 0
 
 
-def _session_from_oracle(cs, o):
+def _session_from_oracle(cs, o, precision=0):
     from cfd_amd.solver import Session
-    S = Session(cs)
+    S = Session(cs, precision=precision)
     S.set_state(o.state())
     S.set_scalars(DISN2=o.disn2)
     return S
 
 
+@pytest.mark.parametrize("precision", [0, 2])
 @pytest.mark.parametrize("which", ["cyl", "ell"])
-def test_session_teacher_forced(which, request):
-    """Each step from the oracle's state; the GPU step must land within 1e-12."""
+def test_session_teacher_forced(which, precision, request):
+    """Each step from the oracle's state; the GPU step must land within 1e-12:
+    precision 0 = the fused FP64 path (regrouped element arithmetic, patch
+    sums), 2 = the FP64 path in the reference's operation and accumulation
+    order."""
     cs, o0 = request.getfixturevalue(which)
     o = OracleSolver(cs)
     o.set_state(o0.state())
     o.disn2 = o0.disn2
-    S = _session_from_oracle(cs, o)
+    S = _session_from_oracle(cs, o, precision)
     for k in range(3):
         st = o.state()
         S.set_state(st)
