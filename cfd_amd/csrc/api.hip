@@ -1113,11 +1113,12 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
             std::vector<int> inp(3 * (size_t)ne);
             m->t.inpoel.download(inp.data(), inp.size(), s);
             CFD_HIP(hipStreamSynchronize(s));
-            if (d->precision == 1)
-                build_patches(S->patches, inp.data(), d->X, d->Y, ne, np, kPatchElems, 256, s);
-            else
-                build_patches(S->patches, inp.data(), d->X, d->Y, ne, np, kPatchElems64,
-                              kPatchElems64, s);
+            const int pe = d->patch_elems > 0 ? d->patch_elems
+                                              : (d->precision == 1 ? kPatchElems : kPatchElems64);
+            const int blk = d->patch_block > 0 ? d->patch_block : 256;
+            CFD_ARG(pe <= 682, "patch_elems must be at most 682");
+            CFD_ARG(blk % 64 == 0 && blk <= 512, "patch_block must be a multiple of 64 up to 512");
+            build_patches(S->patches, inp.data(), d->X, d->Y, ne, np, pe, blk, s);
         } else {
             S->c_calc.alloc(12 * (size_t)ne);
             S->c_fuente.alloc(12 * (size_t)ne);

@@ -235,6 +235,7 @@ void launch_stage_update(const Tables& t, const Consts& c, const BcTable& bc, do
 // node's slot range (lptr); per halo node its partial-sum entries in
 // ascending patch order.
 // ---------------------------------------------------------------------------
+// default patch shapes, 256 threads each (measured on the 1024 x 2048 O-grid)
 constexpr int kPatchElems = 512;     // FP32 path: float4 slots
 constexpr int kPatchElems64 = 384;   // FP64 path: double4 slots
 constexpr int kPatchMaxNodes = 700;  // keeps the fused kernel's LDS within 64 KB

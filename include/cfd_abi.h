@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 2
+#define CFD_ABI_VERSION 3
 
 enum {
     CFD_OK = 0,
@@ -211,6 +211,11 @@ typedef struct {
                                    correction, trained online on the solver's
                                    own solutions) when its initial residual is
                                    smaller, else the previous solution       */
+    int patch_elems;         /* fused paths (precision 0, 1): elements per
+                                patch, 1..682 within the kernel's 64 KB of
+                                LDS (0: 512 FP32, 384 FP64)                   */
+    int patch_block;         /* threads per patch, a multiple of 64 up to 512
+                                (0: 256)                                     */
 } cfd_case_desc;
 
 enum {
