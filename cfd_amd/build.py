@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libcfd_amd.so")
 SOURCES = ["tables.hip", "geom.hip", "flow.hip", "pcg.hip", "patches.hip", "mlp.hip",
-           "warmstart.hip", "api.hip"]
+           "warmstart.hip", "state.hip", "batch.hip", "api.hip"]
 HEADERS = ["cfd_internal.hpp", "reduce.hpp", "crmath.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

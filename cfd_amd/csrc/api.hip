@@ -17,40 +17,12 @@ namespace cfd {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
-static Consts to_consts(const cfd_consts* c) {
-    Consts k;
-    k.gama = c->gama; k.fr = c->fr; k.fmu = c->fmu; k.fk = c->fk; k.fcv = c->fcv;
-    k.t_inf = c->t_inf; k.rho_inf = c->rho_inf; k.u_inf = c->u_inf; k.v_inf = c->v_inf;
-    k.cte = c->cte; k.fsafe = c->fsafe;
-    return k;
-}
+static Consts to_consts(const cfd_consts* c) { return consts_of(c); }
 
 // ---------------------------------------------------------------------------
 // small kernels used by the entries and the session
 // ---------------------------------------------------------------------------
 namespace {
-// free stream (ns2DComp.ALE.f90:409-420) plus the start values of the work arrays
-__global__ void k_freestream(double* __restrict__ U, double* __restrict__ U1, double* __restrict__ VX,
-                             double* __restrict__ VY, double* __restrict__ T, double* __restrict__ GAMM,
-                             double* __restrict__ RHO, double* __restrict__ E, double* __restrict__ WX,
-                             double* __restrict__ WY, double rho, double u, double v, double t,
-                             double gama, double pamb, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    GAMM[i] = gama;
-    double energia = pamb / ((gama - 1.0) * rho) + 0.5 * (u * u + v * v);
-    double u4 = energia * rho;
-    double4 q = make_double4(rho, rho * u, rho * v, u4);
-    reinterpret_cast<double4*>(U)[i] = q;
-    reinterpret_cast<double4*>(U1)[i] = q;
-    VX[i] = u;
-    VY[i] = v;
-    T[i] = t;
-    RHO[i] = rho;
-    E[i] = u4 / rho;
-    WX[i] = -0.0;
-    WY[i] = 0.0;
-}
 // POS_AUX(1:NMOVE) = DPOS(ilaux), POS_AUX(NMOVE+1:NNMOVE) = 0 (meshMove.f90:79-89)
 __global__ void k_pos_aux(const int* __restrict__ ilaux, const double* __restrict__ dpos,
                           double* __restrict__ pos, int nmove, int nn) {
@@ -70,159 +42,11 @@ __global__ void k_move(double* __restrict__ X, double* __restrict__ X1, double* 
     W[i] = d / dtmin;
 }
 
-// Session scalars live on the device: the CFD_S_* slots, then work slots.
-enum { X_RAW = CFD_NSCALARS, X_FDTL, X_CA, X_SA, X_NCTL };
-
-// Step control (ns2DComp.ALE.f90:142-166, meshMove.f90:55-70): DTMIN from
-// the deltat partials, the 1 % hysteresis against DTMIN1, the clock, the
-// local-time-step blend factor and the body's rotation increment for TRANSF.
-// BANDERA's increment after fluidStructure (ns2DComp.ALE.f90:250) is applied
-// here too: nothing in between reads it.
-__global__ void __launch_bounds__(256)
-k_step_ctl(const double* __restrict__ partials, int nb, double* __restrict__ ctl, int itlocal,
-           int motion, double amp, double omega) {
-    const double m = combine_partials(partials, nb, 1.0e20, OpMin());
-    if (threadIdx.x != 0) return;
-    double dtmin = m;
-    ctl[X_RAW] = dtmin;
-    const double iter = ctl[CFD_S_ITER] + 1.0;
-    ctl[CFD_S_ITER] = iter;
-    double bandera = ctl[CFD_S_BANDERA];
-    double dtmin1 = ctl[CFD_S_DTMIN1];
-    if ((int)bandera == 1) {
-        dtmin1 = dtmin;
-        bandera = 2.0;
-    }
-    const double porc = fabs((dtmin - dtmin1) / dtmin);
-    if (100.0 * porc <= 1.0) {
-        dtmin = dtmin1;
-    } else {
-        dtmin1 = dtmin;
-        bandera = 2.0;
-    }
-    ctl[CFD_S_DTMIN1] = dtmin1;
-    ctl[CFD_S_BANDERA] = bandera + 1.0;
-    ctl[CFD_S_DTMIN] = dtmin;
-    ctl[X_FDTL] = itlocal != 0 ? 1.0 - exp(-iter * 4.6 / itlocal) : 1.0;
-    const double time = ctl[CFD_S_TIME] + dtmin;
-    ctl[CFD_S_TIME] = time;
-    // meshMove.f90:55-57,70: ampli*sin(10 t), ampli = pi/8 (motion 0)
-    const double pi = 3.141592653589793;
-    const double ang = motion == 1 ? amp * sin(omega * time) : pi / 8.0 * sin(10.0 * time);
-    const double alpha = ang - ctl[CFD_S_DISN2];
-    ctl[CFD_S_DISN2] = ang;
-    ctl[X_CA] = cos(alpha);
-    ctl[X_SA] = sin(alpha);
-}
-
-// DT clipped at 10*min(DT) (subrutinas.f90:210-216), then DTL = DTMIN, or
-// DTMIN*f + DT*(1-f) with local time stepping (ns2DComp.ALE.f90:158-164)
-__global__ void k_dt_finish(double* __restrict__ DT, double* __restrict__ DTL,
-                            const double* __restrict__ ctl, int blend, int n) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const double cota = 10.0 * ctl[X_RAW], dtmin = ctl[CFD_S_DTMIN];
-    double d = DT[e];
-    if (d > cota) {
-        d = cota;
-        DT[e] = d;
-    }
-    if (blend) {
-        const double f = ctl[X_FDTL];
-        DTL[e] = dtmin * f + d * (1.0 - f);
-    } else {
-        DTL[e] = dtmin;
-    }
-}
-
 __global__ void k_put2(double* __restrict__ a, double va, double* __restrict__ b, double vb) {
     *a = va;
     *b = vb;
 }
 }  // namespace
-
-// ---------------------------------------------------------------------------
-// Boundary-condition table (host construction, reference list semantics)
-// ---------------------------------------------------------------------------
-struct BcHost {
-    std::vector<int> nodes, bcidx, flags;
-    std::vector<double> vals;
-    std::vector<int> wnode_bc, wnode_ptr, wnode_edges;  // wall-node incidence
-};
-
-static BcHost build_bc_host(int npoin, const cfd_bc_lists* bc) {
-    BcHost h;
-    h.bcidx.assign(npoin, -1);
-    auto entry = [&](int node) -> int {
-        CFD_CHECK(node >= 1 && node <= npoin, "boundary list node id out of range");
-        int& b = h.bcidx[node - 1];
-        if (b < 0) {
-            b = (int)h.nodes.size();
-            h.nodes.push_back(node);
-            h.flags.push_back(0);
-            h.vals.insert(h.vals.end(), 8, 0.0);
-        }
-        return b;
-    };
-    if (!bc) return h;
-    for (int i = 0; i < bc->nfixv; ++i) {           // FIXVEL: last occurrence wins
-        int b = entry(bc->ifixv_node[i]);
-        h.flags[b] |= 1;
-        h.vals[8 * b + 0] = bc->rfixv_valuex[i];
-        h.vals[8 * b + 1] = bc->rfixv_valuey[i];
-    }
-    for (int i = 0; i < bc->nfixrho; ++i) {
-        int b = entry(bc->ifixrho_node[i]);
-        h.flags[b] |= 4;
-        h.vals[8 * b + 4] = bc->rfixrho_value[i];
-    }
-    for (int i = 0; i < bc->nfixt; ++i) {
-        int b = entry(bc->ifixt_node[i]);
-        h.flags[b] |= 8;
-        h.vals[8 * b + 5] = bc->rfixt_value[i];
-    }
-    // wall nodes: incident edges in ascending edge order (NORMALES sums)
-    std::map<int, std::vector<int>> inc;
-    for (int e = 0; e < bc->nwall; ++e)
-        for (int j = 0; j < 2; ++j) inc[bc->wall[2 * e + j]].push_back(e);
-    h.wnode_ptr.push_back(0);
-    for (auto& kv : inc) {
-        int b = entry(kv.first);
-        h.wnode_bc.push_back(b);
-        for (int e : kv.second) h.wnode_edges.push_back(e);
-        h.wnode_ptr.push_back((int)h.wnode_edges.size());
-    }
-    return h;
-}
-
-struct BcDevice {
-    BcTable tab;
-    DBuf<int> wnode_bc, wnode_ptr, wnode_edges, wall;
-    int nw = 0;
-    void upload(const BcHost& h, int npoin, const cfd_bc_lists* bc, hipStream_t s) {
-        tab.nb = (int)h.nodes.size();
-        tab.bcidx.alloc(npoin);
-        tab.bcidx.upload(h.bcidx.data(), npoin, s);
-        tab.flags.alloc(std::max<size_t>(1, h.flags.size()));
-        tab.flags.upload(h.flags.data(), h.flags.size(), s);
-        tab.vals.alloc(std::max<size_t>(8, h.vals.size()));
-        tab.vals.upload(h.vals.data(), h.vals.size(), s);
-        nw = (int)h.wnode_bc.size();
-        wnode_bc.alloc(std::max(1, nw));
-        wnode_bc.upload(h.wnode_bc.data(), nw, s);
-        wnode_ptr.alloc(nw + 1);
-        wnode_ptr.upload(h.wnode_ptr.data(), nw + 1, s);
-        wnode_edges.alloc(std::max<size_t>(1, h.wnode_edges.size()));
-        wnode_edges.upload(h.wnode_edges.data(), h.wnode_edges.size(), s);
-        int nwall = bc ? bc->nwall : 0;
-        wall.alloc(std::max(2, 2 * nwall));
-        if (nwall) wall.upload(bc->wall, 2 * nwall, s);
-        CFD_HIP(hipStreamSynchronize(s));
-    }
-    void normales(const double* X, const double* Y, hipStream_t s) {
-        launch_normales(tab, nw, wnode_bc.p, wnode_ptr.p, wnode_edges.p, wall.p, X, Y, s);
-    }
-};
 
 }  // namespace cfd
 
@@ -926,10 +750,9 @@ struct cfd_session {
         tb(CFD_K_DELTAT);
         const int nb = launch_deltat_partials(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p, DT.p,
                                               partials.p, s);
-        k_step_ctl<<<1, 256, 0, s>>>(partials.p, nb, ctl.p, d.itlocal, d.motion, d.motion_amp,
-                                     d.motion_omega);
-        k_dt_finish<<<grid_for(ne), kBlock, 0, s>>>(DT.p, DTL.p, ctl.p, d.itlocal != 0, ne);
-        CFD_HIP(hipGetLastError());
+        launch_step_ctl(partials.p, nullptr, nb, ctl.p, 1, d.itlocal, d.motion, d.motion_amp,
+                        d.motion_omega, s);
+        launch_dt_finish(DT.p, DTL.p, ctl.p, nullptr, d.itlocal != 0, ne, s);
         te(CFD_K_DELTAT);
         const double* dtmin = scal(CFD_S_DTMIN);
 
@@ -1166,14 +989,9 @@ int cfd_session_destroy(cfd_session* S) {
 int cfd_session_init_freestream(cfd_session* S) {
     CFD_TRY({
         CFD_ARG(S, "null session");
-        const Consts& c = S->c;
-        const int np = S->np;
-        double pamb = c.rho_inf * c.fr * c.t_inf;
-        k_freestream<<<grid_for(np), kBlock, 0, S->s>>>(S->U(), S->U1(), S->VX.p, S->VY.p, S->T.p,
-                                                        S->GAMM.p, S->RHO.p, S->E.p, S->WX.p, S->WY.p,
-                                                        c.rho_inf, c.u_inf, c.v_inf, c.t_inf, c.gama,
-                                                        pamb, np);
-        CFD_HIP(hipGetLastError());
+        launch_freestream({S->U(), S->U1(), S->VX.p, S->VY.p, S->T.p, S->GAMM.p, S->RHO.p, S->E.p,
+                           S->WX.p, S->WY.p},
+                          S->c, S->np, S->s);
         S->geometry();
         CFD_HIP(hipMemcpyAsync(S->AREA_OLD.p, S->AREA.p, sizeof(double) * S->ne,
                                hipMemcpyDeviceToDevice, S->s));

@@ -156,6 +156,12 @@ constexpr int kRedBlocks = 2048;
 int launch_deltat_partials(const Tables& t, const Consts& c, const double* area, const double* T,
                            const double* VX, const double* VY, const double* WX, const double* WY,
                            double* DT, double* partials, hipStream_t s);
+// The same with block b over the elements [chunk[b], chunk[b+1]): one min
+// partial per chunk (batches keep each chunk inside one case)
+void launch_deltat_chunks(const Tables& t, const Consts& c, const double* area, const double* T,
+                          const double* VX, const double* VY, const double* WX, const double* WY,
+                          double* DT, const int* chunk, int nchunks, double* partials,
+                          const int* chunk_case, const Consts* cases, hipStream_t s);
 // Inputs of the patch element residual (nelem values each, patch order, of
 // the element type R: float on the FP32 path, double on the fused FP64
 // path), written by ESTAB in place of its element-order outputs: nu =
@@ -166,6 +172,11 @@ struct ElemP {
     const double *area = nullptr, *dtl = nullptr;
     const int* perm = nullptr;
     double cte = 1.0;
+    // batches: element e belongs to case ecase[e]; its DTMIN is
+    // dtmin[ecase[e] * dt_stride] and its constants cases[ecase[e]]
+    const int* ecase = nullptr;
+    const struct Consts* cases = nullptr;
+    int dt_stride = 0;
 };
 using Elem32 = ElemP<float>;
 using Elem64 = ElemP<double>;
@@ -242,8 +253,14 @@ constexpr int kPatchMaxNodes = 700;  // keeps the fused kernel's LDS within 64 K
 struct PatchSet {
     int npatch = 0, max_nodes = 0, nentries = 0, nhalo = 0;
     int pe = kPatchElems, block = 256;  // elements per patch, threads per patch
+    bool persistent = false;            // a resident grid loops over the patches
+    int grid[2] = {0, 0};               // its size, intermediate and final stages
+    // batches: case of each patch and of each halo node, the cases' constants
+    DBuf<int> pcase, hcase;
+    const struct Consts* cases = nullptr;
     DBuf<int> perm, inv_perm;      // patch position <-> element (0-based)
     DBuf<uint32_t> loc;            // 2 per element (patch order)
+    DBuf<int> elem_base;           // npatch+1: patch p holds positions [elem_base[p], [p+1])
     DBuf<int> node_base, n_int, halo_base;
     DBuf<int> pnode;               // 1-based node ids, per patch
     DBuf<uint16_t> lptr;           // nl+1 per patch, at node_base[p] + p
@@ -252,6 +269,11 @@ struct PatchSet {
 };
 void build_patches(PatchSet& ps, const int* inpoel_host, const double* X_host,
                    const double* Y_host, int nelem, int npoin, int pe, int block, hipStream_t s);
+// A batch of disjoint meshes numbered one after the other: each case's
+// patches as build_patches makes them, concatenated
+void build_patches_batch(PatchSet& ps, int ncases, const int* const* inpoel_host,
+                         const double* const* X_host, const double* const* Y_host,
+                         const int* nelem, const int* npoin, int pe, int block, hipStream_t s);
 
 // Fused element residual + RK stage for the patch-interior nodes, then the
 // stage for the halo nodes (subrutinas.f90:679-826 with calcRHS.f90:37-141
@@ -267,6 +289,14 @@ template <class R>
 void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, const BcTable& bc,
                         double rk_fact, const PatchIn<R>& in, const StageNodes& nd,
                         bool final_stage, hipStream_t s);
+// A resident grid for ps (persistent = true): as many blocks of the fused
+// kernel (element type R) as stay resident on the device at once
+template <class R>
+void patch_make_persistent(PatchSet& ps, bool fuente);
+// whether the fused kernel's LDS (node values, W, slots of slot_bytes,
+// slot ranges) fits its 64 KB budget for patches of pe elements and up to
+// max_nodes nodes
+bool patch_lds_fits(int max_nodes, int pe, size_t slot_bytes);
 // dNx/dNy in patch order, rounded to R
 template <class R>
 void launch_to_patch(const double* a, R* b, const int* perm, int nelem, hipStream_t s);
@@ -327,6 +357,65 @@ struct WarmStart {
 void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, const int* wnode_ptr,
                      const int* wnode_edges, const int* wall, const double* X, const double* Y,
                      hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Session state helpers (state.hip), shared by the session and the batch
+// ---------------------------------------------------------------------------
+// Session scalars live on the device: the CFD_S_* slots, then work slots.
+enum { X_RAW = CFD_NSCALARS, X_FDTL, X_CA, X_SA, X_NCTL };
+
+// Boundary-condition table (host construction, reference list semantics)
+struct BcHost {
+    std::vector<int> nodes, bcidx, flags;
+    std::vector<double> vals;
+    std::vector<int> wnode_bc, wnode_ptr, wnode_edges;  // wall-node incidence
+};
+BcHost build_bc_host(int npoin, const cfd_bc_lists* bc);
+
+struct BcDevice {
+    BcTable tab;
+    DBuf<int> wnode_bc, wnode_ptr, wnode_edges, wall;
+    int nw = 0;
+    void upload(const BcHost& h, int npoin, const cfd_bc_lists* bc, hipStream_t s) {
+        tab.nb = (int)h.nodes.size();
+        tab.bcidx.alloc(npoin);
+        tab.bcidx.upload(h.bcidx.data(), npoin, s);
+        tab.flags.alloc(std::max<size_t>(1, h.flags.size()));
+        tab.flags.upload(h.flags.data(), h.flags.size(), s);
+        tab.vals.alloc(std::max<size_t>(8, h.vals.size()));
+        tab.vals.upload(h.vals.data(), h.vals.size(), s);
+        nw = (int)h.wnode_bc.size();
+        wnode_bc.alloc(std::max(1, nw));
+        wnode_bc.upload(h.wnode_bc.data(), nw, s);
+        wnode_ptr.alloc(nw + 1);
+        wnode_ptr.upload(h.wnode_ptr.data(), nw + 1, s);
+        wnode_edges.alloc(std::max<size_t>(1, h.wnode_edges.size()));
+        wnode_edges.upload(h.wnode_edges.data(), h.wnode_edges.size(), s);
+        int nwall = bc ? bc->nwall : 0;
+        wall.alloc(std::max(2, 2 * nwall));
+        if (nwall) wall.upload(bc->wall, 2 * nwall, s);
+        CFD_HIP(hipStreamSynchronize(s));
+    }
+    void normales(const double* X, const double* Y, hipStream_t s) {
+        launch_normales(tab, nw, wnode_bc.p, wnode_ptr.p, wnode_edges.p, wall.p, X, Y, s);
+    }
+};
+
+// free stream (ns2DComp.ALE.f90:409-420) plus the start values of the work
+// arrays, for n nodes from the given pointers
+struct NodeState {
+    double *U, *U1, *VX, *VY, *T, *GAMM, *RHO, *E, *WX, *WY;
+};
+void launch_freestream(const NodeState& ns, const Consts& c, int n, hipStream_t s);
+// the C-ABI constants as the kernels take them
+Consts consts_of(const cfd_consts* c);
+// Step control: one block per case (ncases scalar blocks of X_NCTL in ctl);
+// case c combines partials [pb[c], pb[c+1]) (all nb when pb is null)
+void launch_step_ctl(const double* partials, const int* pb, int nb, double* ctl, int ncases,
+                     int itlocal, int motion, double amp, double omega, hipStream_t s);
+// DT clip and DTL; element e reads case ecase[e] (case 0 when null)
+void launch_dt_finish(double* DT, double* DTL, const double* ctl, const int* ecase, bool blend,
+                      int nelem, hipStream_t s);
 
 // meshMove.f90 FORCES / TRANSF / the node updates of fluidStructure
 void launch_forces(int nedge, const int* iset, const double* X, const double* Y, const double* P,

@@ -30,9 +30,21 @@ __global__ void __launch_bounds__(kBlock)
 k_deltat(const int* __restrict__ inpoel, const double* __restrict__ area, const double* __restrict__ T,
          const double* __restrict__ VX, const double* __restrict__ VY, const double* __restrict__ WX,
          const double* __restrict__ WY, double* __restrict__ DT, int nelem, double t_inf,
-         double fsafe, double* __restrict__ partials) {
+         double fsafe, double* __restrict__ partials, const int* __restrict__ chunk,
+         const int* __restrict__ chunk_case, const Consts* __restrict__ cases) {
+    // grid-stride over all elements, or (chunk) block b over [chunk[b], chunk[b+1])
+    // with (cases) the constants of the chunk's case
+    if (chunk && cases) {
+        const Consts& q = cases[chunk_case[blockIdx.x]];
+        t_inf = q.t_inf;
+        fsafe = q.fsafe;
+    }
+    const int beg = chunk ? chunk[blockIdx.x] + (int)threadIdx.x
+                          : (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int end = chunk ? chunk[blockIdx.x + 1] : nelem;
+    const int step = chunk ? (int)blockDim.x : (int)(gridDim.x * blockDim.x);
     double dmin = 1.0e20;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += gridDim.x * blockDim.x) {
+    for (int e = beg; e < end; e += step) {
         const int* ip = inpoel + 3 * e;
         double vumax = 0.0, vvmax = 0.0;
         int p0 = ip[0] - 1, p1 = ip[1] - 1, p2 = ip[2] - 1;
@@ -80,13 +92,17 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
         const double* __restrict__ GAMM, const double* __restrict__ VEL_X,
         const double* __restrict__ VEL_Y, const double* __restrict__ W_X,
         const double* __restrict__ W_Y, const double* __restrict__ dNx,
-        const double* __restrict__ dNy, int nelem, double fr, double tinf, double rhoinf,
+        const double* __restrict__ dNy, int nelem, double fr_, double tinf_, double rhoinf_,
         const double* __restrict__ dtminp, double* __restrict__ shoc, double* __restrict__ t1,
         double* __restrict__ t2, double* __restrict__ t3, ElemP<R> o) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nelem) return;
     const int e = kPatch ? o.perm[q] : q;
-    const double dtmin = *dtminp;
+    const int cs = (kPatch && o.ecase) ? o.ecase[e] : 0;
+    const double dtmin = dtminp[(size_t)cs * o.dt_stride];
+    const Consts* cq = (kPatch && o.cases) ? o.cases + cs : nullptr;
+    const double fr = cq ? cq->fr : fr_, tinf = cq ? cq->t_inf : tinf_;
+    const double rinf = cq ? cq->rho_inf : rhoinf_, cte = cq ? cq->cte : o.cte;
     int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
     double DNX[3] = {dNx[3 * e], dNx[3 * e + 1], dNx[3 * e + 2]};
     double DNY[3] = {dNy[3 * e], dNy[3 * e + 1], dNy[3 * e + 2]};
@@ -144,13 +160,13 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
     double rrr = pow_m1_2(resumen);
     double s2 = rrr, s3 = rrr;
     if (fmu != 0.0) {
-        double ts3 = (h_rgn * h_rgn) / (4.0 * fmu / rhoinf);
-        double ts3e = (h_rgne * h_rgne) / (4.0 * fmu / rhoinf);
+        double ts3 = (h_rgn * h_rgn) / (4.0 * fmu / rinf);
+        double ts3e = (h_rgne * h_rgne) / (4.0 * fmu / rinf);
         s2 = pow_m1_2(resumen + 1.0 / (ts3 * ts3));
         s3 = pow_m1_2(resumen + 1.0 / (ts3e * ts3e));
     }
     if (kPatch) {  // inputs of the fused element residual, rounded once here
-        o.nu[q] = (R)(sh * o.cte);
+        o.nu[q] = (R)(sh * cte);
         o.ta[q] = (R)rrr;
         o.tb[q] = (R)s2;
         o.tc[q] = (R)s3;
@@ -709,10 +725,12 @@ __device__ inline void elem_residual(const ElemConstsP<R>& kc, const double4 u0,
 // halo nodes store their partial sum.
 struct PatchDev {
     const uint32_t* loc;
-    const int *node_base, *n_int, *halo_base, *pnode;
+    const int *elem_base, *node_base, *n_int, *halo_base, *pnode;
     const uint16_t* lptr;
     double* hpart;
-    int nelem, maxn, pe;
+    int nelem, maxn, pe, npatch;
+    const int* pcase;      // batches: case of each patch (null: kc and a as given)
+    const Consts* cases;
 };
 
 template <class R, bool kFuente, bool kFinal>
@@ -726,56 +744,71 @@ k_patch_stage(PatchDev pd, ElemConstsP<R> kc, PatchIn<R> in, StageDev a) {
     V4* sC = reinterpret_cast<V4*>(sW + (kFuente ? maxn : 0));    // 3*pe
     double* sT = reinterpret_cast<double*>(sC + 3 * pe);          // maxn
     uint16_t* sP = reinterpret_cast<uint16_t*>(sT + maxn);        // maxn+1
-    const int p = blockIdx.x;
-    const int nb = pd.node_base[p], nl = pd.node_base[p + 1] - nb, nint = pd.n_int[p];
-    const long long e0 = (long long)p * pe;
-    const int nep = (int)min((long long)pe, pd.nelem - e0);
-    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-        const int g = pd.pnode[nb + l] - 1;
-        sU[l] = ld4(in.U + 4 * (long long)g);
-        sT[l] = in.T[g];
-        if (kFuente) sW[l] = make_double2(in.WX[g], in.WY[g]);
-    }
-    for (int l = threadIdx.x; l <= nl; l += blockDim.x) sP[l] = pd.lptr[nb + p + l];
-    __syncthreads();
-    for (int q = threadIdx.x; q < nep; q += blockDim.x) {
-        const long long e = e0 + q;
-        const uint64_t w = (uint64_t)pd.loc[2 * e] | ((uint64_t)pd.loc[2 * e + 1] << 32);
-        const int ln[3] = {(int)(w & 1023u), (int)((w >> 10) & 1023u), (int)((w >> 20) & 1023u)};
-        const int ls[3] = {(int)((w >> 30) & 2047u), (int)((w >> 41) & 2047u),
-                           (int)((w >> 52) & 2047u)};
-        const R Nx[3] = {in.dNx[3 * e], in.dNx[3 * e + 1], in.dNx[3 * e + 2]};
-        const R Ny[3] = {in.dNy[3 * e], in.dNy[3 * e + 1], in.dNy[3 * e + 2]};
-        const R tau[3] = {in.ta[e], in.tb[e], in.tc[e]};
-        const double T3[3] = {sT[ln[0]], sT[ln[1]], sT[ln[2]]};
-        double wx3[3] = {0.0, 0.0, 0.0}, wy3[3] = {0.0, 0.0, 0.0};
-        if (kFuente) {
+    // one patch per block, or (resident grid) patches blockIdx.x + k gridDim.x
+    for (int p = blockIdx.x; p < pd.npatch; p += gridDim.x) {
+        ElemConstsP<R> kl = kc;
+        StageDev al = a;
+        if (pd.pcase) {  // the constants of this patch's case, rounded as the host rounds them
+            const Consts& q = pd.cases[pd.pcase[p]];
+            kl.g = (R)q.gama;
+            kl.Cv = (R)q.fcv;
+            kl.lambda_ref = (R)q.fk;
+            kl.mu_ref = (R)q.fmu;
+            kl.T_inf = (R)q.t_inf;
+            kl.viscous = q.fmu > 2.2250738585072014e-308 ? 1 : 0;
+            al.fr = q.fr;
+        }
+        const int nb = pd.node_base[p], nl = pd.node_base[p + 1] - nb, nint = pd.n_int[p];
+        const long long e0 = pd.elem_base[p];
+        const int nep = pd.elem_base[p + 1] - pd.elem_base[p];
+        for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+            const int g = pd.pnode[nb + l] - 1;
+            sU[l] = ld4(in.U + 4 * (long long)g);
+            sT[l] = in.T[g];
+            if (kFuente) sW[l] = make_double2(in.WX[g], in.WY[g]);
+        }
+        for (int l = threadIdx.x; l <= nl; l += blockDim.x) sP[l] = pd.lptr[nb + p + l];
+        __syncthreads();
+        for (int q = threadIdx.x; q < nep; q += blockDim.x) {
+            const long long e = e0 + q;
+            const uint64_t w = (uint64_t)pd.loc[2 * e] | ((uint64_t)pd.loc[2 * e + 1] << 32);
+            const int ln[3] = {(int)(w & 1023u), (int)((w >> 10) & 1023u), (int)((w >> 20) & 1023u)};
+            const int ls[3] = {(int)((w >> 30) & 2047u), (int)((w >> 41) & 2047u),
+                               (int)((w >> 52) & 2047u)};
+            const R Nx[3] = {in.dNx[3 * e], in.dNx[3 * e + 1], in.dNx[3 * e + 2]};
+            const R Ny[3] = {in.dNy[3 * e], in.dNy[3 * e + 1], in.dNy[3 * e + 2]};
+            const R tau[3] = {in.ta[e], in.tb[e], in.tc[e]};
+            const double T3[3] = {sT[ln[0]], sT[ln[1]], sT[ln[2]]};
+            double wx3[3] = {0.0, 0.0, 0.0}, wy3[3] = {0.0, 0.0, 0.0};
+            if (kFuente) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                wx3[k] = sW[ln[k]].x;
-                wy3[k] = sW[ln[k]].y;
+                for (int k = 0; k < 3; ++k) {
+                    wx3[k] = sW[ln[k]].x;
+                    wy3[k] = sW[ln[k]].y;
+                }
             }
-        }
-        V4 out[3];
-        elem_residual<R, kFuente>(kc, sU[ln[0]], sU[ln[1]], sU[ln[2]], T3, wx3, wy3, Nx, Ny,
-                                  in.fac[e], in.nu[e], tau, out);
+            V4 out[3];
+            elem_residual<R, kFuente>(kl, sU[ln[0]], sU[ln[1]], sU[ln[2]], T3, wx3, wy3, Nx, Ny,
+                                      in.fac[e], in.nu[e], tau, out);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) sC[ls[k]] = out[k];
-    }
-    __syncthreads();
-    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-        double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-        for (int s = sP[l]; s < sP[l + 1]; ++s) {
-            const V4 c = sC[s];
-            r0 = r0 + (double)c.x;
-            r1 = r1 + (double)c.y;
-            r2 = r2 + (double)c.z;
-            r3 = r3 + (double)c.w;
+            for (int k = 0; k < 3; ++k) sC[ls[k]] = out[k];
         }
-        if (l < nint)
-            stage_node<kFinal>(a, pd.pnode[nb + l] - 1, sU[l], r0, r1, r2, r3);
-        else
-            st4(pd.hpart + 4 * (long long)(pd.halo_base[p] + l - nint), r0, r1, r2, r3);
+        __syncthreads();
+        for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+            double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+            for (int s = sP[l]; s < sP[l + 1]; ++s) {
+                const V4 c = sC[s];
+                r0 = r0 + (double)c.x;
+                r1 = r1 + (double)c.y;
+                r2 = r2 + (double)c.z;
+                r3 = r3 + (double)c.w;
+            }
+            if (l < nint)
+                stage_node<kFinal>(al, pd.pnode[nb + l] - 1, sU[l], r0, r1, r2, r3);
+            else
+                st4(pd.hpart + 4 * (long long)(pd.halo_base[p] + l - nint), r0, r1, r2, r3);
+        }
+        __syncthreads();  // the next patch overwrites the LDS arrays
     }
 }
 
@@ -784,9 +817,11 @@ template <bool kFinal>
 __global__ void __launch_bounds__(kBlock)
 k_halo_stage(const int* __restrict__ hnode, const int* __restrict__ hptr,
              const int* __restrict__ hidx, const double* __restrict__ hpart,
-             const double* __restrict__ U, StageDev a, int nhalo) {
+             const double* __restrict__ U, StageDev a, int nhalo, const int* __restrict__ hcase,
+             const Consts* __restrict__ cases) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nhalo) return;
+    if (hcase) a.fr = cases[hcase[h]].fr;  // batches: the node's case
     const int g = hnode[h] - 1;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     for (int j = hptr[h]; j < hptr[h + 1]; ++j) {
@@ -818,9 +853,18 @@ int launch_deltat_partials(const Tables& t, const Consts& c, const double* area,
                            double* DT, double* partials, hipStream_t s) {
     const unsigned g = std::min<unsigned>(grid_for(t.nelem), kRedBlocks);
     k_deltat<<<g, kBlock, 0, s>>>(t.inpoel.p, area, T, VX, VY, WX, WY, DT, t.nelem, c.t_inf, c.fsafe,
-                                  partials);
+                                  partials, nullptr, nullptr, nullptr);
     CFD_HIP(hipGetLastError());
     return (int)g;
+}
+
+void launch_deltat_chunks(const Tables& t, const Consts& c, const double* area, const double* T,
+                          const double* VX, const double* VY, const double* WX, const double* WY,
+                          double* DT, const int* chunk, int nchunks, double* partials,
+                          const int* chunk_case, const Consts* cases, hipStream_t s) {
+    k_deltat<<<nchunks, kBlock, 0, s>>>(t.inpoel.p, area, T, VX, VY, WX, WY, DT, t.nelem, c.t_inf,
+                                        c.fsafe, partials, chunk, chunk_case, cases);
+    CFD_HIP(hipGetLastError());
 }
 
 void launch_deltat(const Tables& t, const Consts& c, const double* area, const double* T,
@@ -969,14 +1013,20 @@ static size_t patch_lds_bytes(int maxn, bool fuente, int pe, size_t slot) {
            3 * (size_t)pe * slot + (size_t)(maxn + 1) * sizeof(uint16_t);
 }
 
+bool patch_lds_fits(int max_nodes, int pe, size_t slot_bytes) {
+    return patch_lds_bytes(max_nodes, true, pe, slot_bytes) <= 65536;
+}
+
 template <class R, bool kFuente, bool kFinal>
 static void patch_launch_t(const PatchSet& ps, const PatchDev& pd, const ElemConstsP<R>& kc,
                            const PatchIn<R>& in, const StageDev& a, hipStream_t s) {
     const size_t lds = patch_lds_bytes(pd.maxn, kFuente, pd.pe, sizeof(typename Vec4<R>::type));
-    k_patch_stage<R, kFuente, kFinal><<<ps.npatch, ps.block, lds, s>>>(pd, kc, in, a);
+    const int grid = ps.persistent && ps.grid[kFinal] > 0 ? ps.grid[kFinal] : ps.npatch;
+    k_patch_stage<R, kFuente, kFinal><<<grid, ps.block, lds, s>>>(pd, kc, in, a);
     if (ps.nhalo > 0)
         k_halo_stage<kFinal><<<grid_for(ps.nhalo), kBlock, 0, s>>>(
-            ps.hnode.p, ps.hptr.p, ps.hidx.p, ps.hpart.p, in.U, a, ps.nhalo);
+            ps.hnode.p, ps.hptr.p, ps.hidx.p, ps.hpart.p, in.U, a, ps.nhalo,
+            ps.cases ? ps.hcase.p : nullptr, ps.cases);
 }
 
 template <class R>
@@ -992,6 +1042,7 @@ void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, co
     kc.viscous = c.fmu > 2.2250738585072014e-308 ? 1 : 0;
     PatchDev pd;
     pd.loc = ps.loc.p;
+    pd.elem_base = ps.elem_base.p;
     pd.node_base = ps.node_base.p;
     pd.n_int = ps.n_int.p;
     pd.halo_base = ps.halo_base.p;
@@ -1001,8 +1052,11 @@ void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, co
     pd.nelem = t.nelem;
     pd.maxn = ps.max_nodes;
     pd.pe = ps.pe;
+    pd.npatch = ps.npatch;
+    pd.pcase = ps.cases ? ps.pcase.p : nullptr;
+    pd.cases = ps.cases;
     CFD_CHECK(ps.block >= 64 && ps.block <= 512 && ps.block % 64 == 0, "internal: patch block size");
-    CFD_CHECK(patch_lds_bytes(pd.maxn, true, ps.pe, sizeof(typename Vec4<R>::type)) <= 65536,
+    CFD_CHECK(patch_lds_fits(pd.maxn, ps.pe, sizeof(typename Vec4<R>::type)),
               "mesh: element patch exceeds the LDS budget");
     const StageDev a = stage_dev(c, bc, rk_fact, nd);
     const bool fu = in.WX != nullptr;
@@ -1012,6 +1066,27 @@ void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, co
     else patch_launch_t<R, false, false>(ps, pd, kc, in, a, s);
     CFD_HIP(hipGetLastError());
 }
+
+template <class R, bool kFuente, bool kFinal>
+static int resident_blocks(const PatchSet& ps) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    CFD_HIP(hipGetDevice(&dev));
+    CFD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t lds = patch_lds_bytes(ps.max_nodes, kFuente, ps.pe, sizeof(typename Vec4<R>::type));
+    CFD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_patch_stage<R, kFuente, kFinal>), ps.block, lds));
+    return std::max(1, std::min(ps.npatch, ncu * std::max(1, per_cu)));
+}
+
+template <class R>
+void patch_make_persistent(PatchSet& ps, bool fuente) {
+    ps.persistent = true;
+    ps.grid[0] = fuente ? resident_blocks<R, true, false>(ps) : resident_blocks<R, false, false>(ps);
+    ps.grid[1] = fuente ? resident_blocks<R, true, true>(ps) : resident_blocks<R, false, true>(ps);
+}
+
+template void patch_make_persistent<float>(PatchSet&, bool);
+template void patch_make_persistent<double>(PatchSet&, bool);
 
 template <class R>
 void launch_to_patch(const double* a, R* b, const int* perm, int nelem, hipStream_t s) {

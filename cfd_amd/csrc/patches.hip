@@ -30,11 +30,17 @@ inline uint64_t spread_bits(uint64_t v) {  // bit i -> bit 2i (32-bit input)
 }
 }  // namespace
 
-void build_patches(PatchSet& ps, const int* inpoel, const double* X, const double* Y, int ne, int np,
-                   int pe, int block, hipStream_t s) {
+// One mesh's patches, host side.  Node and element ids are the mesh's own.
+struct PatchHost {
+    int npatch = 0, max_nodes = 0, nentries = 0;
+    std::vector<int> perm, elem_base, node_base, n_int, halo_base, pnode, hnode, hptr, hidx;
+    std::vector<uint16_t> lptr;
+    std::vector<uint32_t> loc;
+};
+
+static PatchHost patches_host(const int* inpoel, const double* X, const double* Y, int ne, int np,
+                              int pe) {
     CFD_CHECK(ne > 0 && np > 0, "mesh: empty mesh");
-    // local slots are 11-bit and local node ids 10-bit fields of the per-element word
-    CFD_CHECK(pe >= 1 && 3 * pe <= 2047 && kPatchMaxNodes <= 1023, "internal: patch size");
     // Morton order of element centroids over the bounding box
     double xmin = X[0], xmax = X[0], ymin = Y[0], ymax = Y[0];
     for (int i = 1; i < np; ++i) {
@@ -61,11 +67,8 @@ void build_patches(PatchSet& ps, const int* inpoel, const double* X, const doubl
         key[e] = {spread_bits(qx) | (spread_bits(qy) << 1), e};
     }
     std::sort(key.begin(), key.end());
-    std::vector<int> perm(ne), inv(ne);
-    for (int q = 0; q < ne; ++q) {
-        perm[q] = key[q].second;
-        inv[perm[q]] = q;
-    }
+    std::vector<int> perm(ne);
+    for (int q = 0; q < ne; ++q) perm[q] = key[q].second;
     key.clear();
     key.shrink_to_fit();
 
@@ -151,36 +154,122 @@ void build_patches(PatchSet& ps, const int* inpoel, const double* X, const doubl
     std::vector<int> hidx(nentries);
     for (int t = 0; t < nentries; ++t) hidx[hpos[hnode_of_entry[t]]++] = t;  // t ascends with p
 
+    PatchHost h;
+    h.elem_base.resize(npatch + 1);
+    for (int p = 0; p <= npatch; ++p) h.elem_base[p] = std::min(ne, p * pe);
+    h.npatch = npatch;
+    h.max_nodes = max_nodes;
+    h.nentries = nentries;
+    h.perm = std::move(perm);
+    h.node_base = std::move(node_base);
+    h.n_int = std::move(n_int);
+    h.halo_base = std::move(halo_base);
+    h.pnode = std::move(pnode);
+    h.lptr = std::move(lptr);
+    h.loc = std::move(loc);
+    h.hnode = std::move(hnode);
+    h.hptr = std::move(hptr);
+    h.hidx = std::move(hidx);
+    return h;
+}
+
+static void upload_patches(PatchSet& ps, const PatchHost& h, int ne, int pe, int block,
+                           hipStream_t s) {
+    const int npatch = h.npatch, nentries = h.nentries;
+    std::vector<int> inv(ne);
+    for (int q = 0; q < ne; ++q) inv[h.perm[q]] = q;
     ps.npatch = npatch;
     ps.pe = pe;
     ps.block = block;
-    ps.max_nodes = max_nodes;
+    ps.max_nodes = h.max_nodes;
     ps.nentries = nentries;
-    ps.nhalo = (int)hnode.size();
+    ps.nhalo = (int)h.hnode.size();
     ps.perm.alloc(ne);
-    ps.perm.upload(perm.data(), ne, s);
+    ps.perm.upload(h.perm.data(), ne, s);
     ps.inv_perm.alloc(ne);
     ps.inv_perm.upload(inv.data(), ne, s);
     ps.loc.alloc(2 * (size_t)ne);
-    ps.loc.upload(loc.data(), 2 * (size_t)ne, s);
+    ps.loc.upload(h.loc.data(), 2 * (size_t)ne, s);
+    ps.elem_base.alloc(npatch + 1);
+    ps.elem_base.upload(h.elem_base.data(), npatch + 1, s);
     ps.node_base.alloc(npatch + 1);
-    ps.node_base.upload(node_base.data(), npatch + 1, s);
+    ps.node_base.upload(h.node_base.data(), npatch + 1, s);
     ps.n_int.alloc(npatch);
-    ps.n_int.upload(n_int.data(), npatch, s);
+    ps.n_int.upload(h.n_int.data(), npatch, s);
     ps.halo_base.alloc(npatch + 1);
-    ps.halo_base.upload(halo_base.data(), npatch + 1, s);
-    ps.pnode.alloc(pnode.size());
-    ps.pnode.upload(pnode.data(), pnode.size(), s);
-    ps.lptr.alloc(lptr.size());
-    ps.lptr.upload(lptr.data(), lptr.size(), s);
-    ps.hnode.alloc(std::max<size_t>(1, hnode.size()));
-    ps.hnode.upload(hnode.data(), hnode.size(), s);
-    ps.hptr.alloc(hptr.size());
-    ps.hptr.upload(hptr.data(), hptr.size(), s);
+    ps.halo_base.upload(h.halo_base.data(), npatch + 1, s);
+    ps.pnode.alloc(h.pnode.size());
+    ps.pnode.upload(h.pnode.data(), h.pnode.size(), s);
+    ps.lptr.alloc(h.lptr.size());
+    ps.lptr.upload(h.lptr.data(), h.lptr.size(), s);
+    ps.hnode.alloc(std::max<size_t>(1, h.hnode.size()));
+    ps.hnode.upload(h.hnode.data(), h.hnode.size(), s);
+    ps.hptr.alloc(h.hptr.size());
+    ps.hptr.upload(h.hptr.data(), h.hptr.size(), s);
     ps.hidx.alloc(std::max(1, nentries));
-    ps.hidx.upload(hidx.data(), nentries, s);
+    ps.hidx.upload(h.hidx.data(), nentries, s);
     ps.hpart.alloc(4 * (size_t)std::max(1, nentries));
     CFD_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+}
+
+static void check_shape(int pe, int block) {
+    // local slots are 11-bit and local node ids 10-bit fields of the per-element word
+    CFD_CHECK(pe >= 1 && 3 * pe <= 2047 && kPatchMaxNodes <= 1023, "internal: patch size");
+    CFD_CHECK(block >= 64 && block <= 512 && block % 64 == 0, "internal: patch block size");
+}
+
+void build_patches(PatchSet& ps, const int* inpoel, const double* X, const double* Y, int ne, int np,
+                   int pe, int block, hipStream_t s) {
+    check_shape(pe, block);
+    const PatchHost h = patches_host(inpoel, X, Y, ne, np, pe);
+    upload_patches(ps, h, ne, pe, block, s);
+}
+
+void build_patches_batch(PatchSet& ps, int ncases, const int* const* inpoel,
+                         const double* const* X, const double* const* Y, const int* ne,
+                         const int* np, int pe, int block, hipStream_t s) {
+    check_shape(pe, block);
+    // each case's patches as on its own, then concatenated: element, node,
+    // patch, local-node and halo-entry numbers shifted by the cases before
+    PatchHost u;
+    std::vector<int> pcase, hcase;
+    u.elem_base.push_back(0);
+    u.node_base.push_back(0);
+    u.halo_base.push_back(0);
+    u.hptr.push_back(0);
+    long long ne_tot = 0, np_tot = 0;
+    for (int k = 0; k < ncases; ++k) {
+        const PatchHost h = patches_host(inpoel[k], X[k], Y[k], ne[k], np[k], pe);
+        const int E = (int)ne_tot, N = (int)np_tot;
+        const int NB = u.node_base.back(), H = u.halo_base.back();
+        for (int e : h.perm) u.perm.push_back(e + E);
+        u.loc.insert(u.loc.end(), h.loc.begin(), h.loc.end());
+        for (int p = 1; p <= h.npatch; ++p) {
+            u.elem_base.push_back(h.elem_base[p] + E);
+            u.node_base.push_back(h.node_base[p] + NB);
+            u.halo_base.push_back(h.halo_base[p] + H);
+        }
+        u.n_int.insert(u.n_int.end(), h.n_int.begin(), h.n_int.end());
+        for (int n : h.pnode) u.pnode.push_back(n + N);
+        u.lptr.insert(u.lptr.end(), h.lptr.begin(), h.lptr.end());
+        for (int n : h.hnode) u.hnode.push_back(n + N);
+        for (size_t i = 1; i < h.hptr.size(); ++i) u.hptr.push_back(h.hptr[i] + H);
+        for (int t : h.hidx) u.hidx.push_back(t + H);
+        pcase.insert(pcase.end(), h.npatch, k);
+        hcase.insert(hcase.end(), h.hnode.size(), k);
+        u.npatch += h.npatch;
+        u.nentries += h.nentries;
+        u.max_nodes = std::max(u.max_nodes, h.max_nodes);
+        ne_tot += ne[k];
+        np_tot += np[k];
+        CFD_CHECK(ne_tot < (1ll << 31) && np_tot < (1ll << 31), "arg: batch too large");
+    }
+    upload_patches(ps, u, (int)ne_tot, pe, block, s);
+    ps.pcase.alloc(std::max<size_t>(1, pcase.size()));
+    ps.pcase.upload(pcase.data(), pcase.size(), s);
+    ps.hcase.alloc(std::max<size_t>(1, hcase.size()));
+    ps.hcase.upload(hcase.data(), hcase.size(), s);
+    CFD_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace cfd

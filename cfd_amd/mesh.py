@@ -266,6 +266,28 @@ def ellipse_case(nr=512, nt=1024, mach=0.5, re=5000.0, name="ell", moving=1,
                 moving=moving, xref=xref, yref=yref, fsafe=fs, **kw)
 
 
+def sweep_cases(n: int = 256, seed: int = SEED, nr=(32, 256), nt=(64, 512), mach=(0.2, 0.85),
+                alpha_deg=(-5.0, 5.0), re=(50.0, 5000.0), cfl: float = 0.5) -> list:
+    """The batched sweep: n fixed-mesh cylinder cases of uneven size.  Case k
+    draws, from splitmix64(seed + k) far past the mesh jitter's draws, Nr
+    uniform in nr, Ntheta uniform in nt (inclusive), M uniform in mach, the
+    angle of attack uniform in alpha_deg (set through the free-stream
+    direction) and Re log-uniform in re; its O-grid is jittered with seed +
+    k.  FSAFE gives each case an acoustic CFL of cfl (deltat bounds only the
+    advective and viscous rates)."""
+    cases = []
+    for k in range(n):
+        u = 0.5 * (splitmix64_uniform(seed + k, 5, offset=1 << 40) + 1.0)  # [0, 1)
+        nr_k = nr[0] + min(int(u[0] * (nr[1] - nr[0] + 1)), nr[1] - nr[0])
+        nt_k = nt[0] + min(int(u[1] * (nt[1] - nt[0] + 1)), nt[1] - nt[0])
+        m_k = mach[0] + u[2] * (mach[1] - mach[0])
+        a_k = alpha_deg[0] + u[3] * (alpha_deg[1] - alpha_deg[0])
+        re_k = math.exp(math.log(re[0]) + u[4] * (math.log(re[1]) - math.log(re[0])))
+        cases.append(Case(name=f"sweep{k}", mesh=ogrid(nr_k, nt_k, "circle", seed=seed + k),
+                          mach=m_k, re=re_k, alpha_deg=a_k, fsafe=acoustic_fsafe(m_k, cfl)))
+    return cases
+
+
 # ---------------------------------------------------------------------------
 # Reference text format (dataLoader.f90:17-65 and 95-275)
 # ---------------------------------------------------------------------------

@@ -267,6 +267,51 @@ int cfd_session_timing(cfd_session *s, int enable);
 int cfd_session_timing_mask(cfd_session *s, unsigned mask);
 int cfd_session_kernel_time(cfd_session *s, int slot, double *total_ms, long long *launches);
 
+/* ---- batches of independent fixed-mesh cases ----------------------------
+ * No counterpart in the reference, which runs one case per process: a sweep
+ * of cases of uneven mesh size advanced together on one GPU.  Each case
+ * keeps its own step control (DTMIN, hysteresis, clock) and free stream;
+ * every case's results agree bit for bit with the same case run alone in a
+ * session of the same precision and patch shape.  One step of every case is
+ * one replay of a recorded hipGraph (cfd_batch_set_graph(b, 0): plain
+ * launches); per RK stage one resident-grid launch covers every case. */
+typedef struct cfd_batch cfd_batch;
+typedef struct {
+    int nelem, npoin;
+    const int *inpoel;       /* inpoel(3,nelem), 1-based ids of this case     */
+    const double *X, *Y;     /* node coordinates                              */
+    cfd_consts c;            /* this case's constants (a sweep over Mach and
+                                Reynolds numbers changes FR, FCV, FK and the
+                                free stream from case to case)                */
+    cfd_bc_lists bc;         /* this case's boundary lists (its own ids)      */
+} cfd_batch_case;
+/* precision 0 (FP64) or 1 (FP32 element), as cfd_case_desc.precision;
+ * patch_elems / patch_block as there (0: defaults); itlocal as the case
+ * files' ITLOCAL (0: global time step).  Starts from the free stream. */
+int cfd_batch_create(cfd_batch **out, int ncases, const cfd_batch_case *cases, int precision,
+                     int itlocal, int patch_elems, int patch_block);
+int cfd_batch_destroy(cfd_batch *b);
+int cfd_batch_init_freestream(cfd_batch *b);
+int cfd_batch_set_graph(cfd_batch *b, int use_graph);
+int cfd_batch_step(cfd_batch *b, int nsteps);
+int cfd_batch_sync(cfd_batch *b);
+/* one case's part of a field: CFD_F_U, U1, RHS, T, VEL_X, VEL_Y, W_X, W_Y,
+ * X, Y, RHO, E, P, RMACH, M, GAMM (nodes), DTL, DT, SHOC, T_SUGN1..3, AREA,
+ * DNX, DNY, HH, HHX, HHY (elements) */
+int cfd_batch_case_size(cfd_batch *b, int icase, int field, long long *count);
+int cfd_batch_get_field(cfd_batch *b, int icase, int field, double *host);
+/* one case's CFD_S_* scalars (DTMIN, TIME, DTMIN1, BANDERA, ITER) */
+int cfd_batch_get_scalars(cfd_batch *b, int icase, double *scalars /* CFD_NSCALARS */);
+int cfd_batch_info(cfd_batch *b, long long *npoin, long long *nelem, int *npatch,
+                   int *patch_elems, int *patch_block);
+/* Autotuner: times nsteps graph-replayed steps for each candidate patch
+ * shape (patch_elems[i] elements on patch_block[i] threads), writes the
+ * ms per step of each (-1 for a shape whose patches exceed the fused
+ * kernel's 64 KB of LDS), keeps the fastest (*best) and restarts every case
+ * from the free stream. */
+int cfd_batch_autotune(cfd_batch *b, int ncand, const int *patch_elems, const int *patch_block,
+                       int nsteps, double *ms_per_step, int *best);
+
 #ifdef __cplusplus
 }
 #endif
