@@ -142,6 +142,44 @@ def ale_compare(steps: int = 40, warmup: int = 5) -> dict:
     return res
 
 
+def batch_sweep(steps: int = 100, warmup: int = 5, ncases: int = 256) -> dict:
+    """The batched sweep: 256 fixed-mesh cylinder cases of uneven size (Nr in
+    32..256, Ntheta in 64..512, M in [0.2, 0.85], angle of attack in [-5, 5]
+    deg, Re log-uniform in [50, 5000], each at an acoustic CFL of 0.5), FP32
+    element arithmetic with FP64 accumulation, every case advanced by one
+    hipGraph replay per step; the patch shape is chosen by the autotuner on
+    3 timed steps per candidate, then every case restarts from the free
+    stream."""
+    from cfd_amd.batch import Batch
+    from cfd_amd.mesh import sweep_cases
+    cases = sweep_cases(ncases)
+    t0 = time.perf_counter()
+    B = Batch(cases, precision=1)
+    t_create = time.perf_counter() - t0
+    cand = [(512, 256), (640, 320), (384, 192), (640, 256), (512, 512)]
+    t0 = time.perf_counter()
+    best, times = B.autotune(cand, steps=3)
+    t_tune = time.perf_counter() - t0
+    B.step(warmup)
+    B.sync()
+    t0 = time.perf_counter()
+    B.step(steps)
+    B.sync()
+    sec = time.perf_counter() - t0
+    info = B.info()
+    import numpy as np
+    finite = all(bool(np.isfinite(B.get(k, "U")).all()) for k in range(0, ncases, 15))
+    B.close()
+    return {"workload": f"{ncases} cylinder O-grid cases, {info['npoin']} nodes, {info['nelem']} "
+                        f"triangles in all, fixed meshes, FP32 element / FP64 accumulation",
+            "value": 4.0 * info["nelem"] * steps / sec, "unit": "triangle-RHS evals/s",
+            "ms_per_step": sec / steps * 1e3, "steps": steps, "warmup": warmup,
+            "patch_shape": {"elements": best[0], "threads": best[1]},
+            "autotune_ms_per_step": {f"{a}x{b}": t for (a, b), t in times.items()},
+            "create_s": t_create, "autotune_s": t_tune, "npatch": info["npatch"],
+            "states_finite": finite}
+
+
 def read_traffic(name: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), committed under profiles/; None when absent."""
@@ -203,6 +241,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ale", action="store_true", help="skip the ALE learned-vs-plain comparison")
     ap.add_argument("--ale-steps", type=int, default=40)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
+    ap.add_argument("--batch-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the path is a single-GPU kernel set "
@@ -261,6 +301,8 @@ def main():
                     "roofline_frac": v["achieved"] / HBM_PEAK_GBS, "steps": 20, "warmup": 20}
     if not args.no_ale:
         out["ale"] = ale_compare(args.ale_steps)
+    if not args.no_batch:
+        out["batch"] = batch_sweep(args.batch_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cs)
     print(json.dumps(out))

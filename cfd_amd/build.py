@@ -70,6 +70,7 @@ def build_library(force: bool = True, jobs: int = 8) -> str:
 
 FORTRAN = os.path.join(PKG, "fortran")
 ABI_DEMO = os.path.join(OBJ, "abi_demo")
+SESSION_DEMO = os.path.join(OBJ, "session_demo")
 
 
 def build_fortran() -> str | None:
@@ -81,10 +82,11 @@ def build_fortran() -> str | None:
         return None
     os.makedirs(OBJ, exist_ok=True)
     abi_o = os.path.join(OBJ, "cfd_abi_f.o")
-    demo_o = os.path.join(OBJ, "abi_demo_f.o")
     _run([fc, "-O2", "-J", OBJ, "-c", os.path.join(FORTRAN, "cfd_abi.f90"), "-o", abi_o])
-    _run([fc, "-O2", "-J", OBJ, "-c", os.path.join(FORTRAN, "abi_demo.f90"), "-o", demo_o])
-    _run([fc, demo_o, abi_o, "-L" + PKG, "-lcfd_amd", "-Wl,-rpath,$ORIGIN/..", "-o", ABI_DEMO])
+    for prog, exe in (("abi_demo", ABI_DEMO), ("session_demo", SESSION_DEMO)):
+        obj = os.path.join(OBJ, prog + "_f.o")
+        _run([fc, "-O2", "-J", OBJ, "-c", os.path.join(FORTRAN, prog + ".f90"), "-o", obj])
+        _run([fc, obj, abi_o, "-L" + PKG, "-lcfd_amd", "-Wl,-rpath,$ORIGIN/..", "-o", exe])
     return ABI_DEMO
 
 

@@ -1,5 +1,6 @@
 ! Fortran 2003 interface to libcfd_amd.so: the C ABI of include/cfd_abi.h
-! (host-pointer entries) as iso_c_binding interfaces, plus cfd_check, which
+! (host-pointer entries, the device-resident session, batches) as
+! iso_c_binding interfaces, plus cfd_check, which
 ! turns a non-zero return code into ERROR STOP with the library's message.
 ! Arrays follow the reference's conventions: 1-based ids, 0-based offsets,
 ! column-major U(4,npoin), inpoel(3,nelem), dNx(3,nelem).
@@ -25,6 +26,47 @@ module cfd_abi
         integer(c_int) :: nwall
         type(c_ptr) :: wall
     end type cfd_bc_lists
+
+    ! device-resident session: the case (cfd_case_desc of include/cfd_abi.h)
+    type, bind(c) :: cfd_case_desc
+        type(cfd_consts) :: c
+        integer(c_int) :: nrk, moving, itlocal
+        real(c_double) :: xref, yref
+        integer(c_int) :: nsetedges
+        type(c_ptr) :: iset
+        integer(c_int) :: nmove, nfix_move
+        type(c_ptr) :: ilaux
+        type(cfd_bc_lists) :: bc
+        type(c_ptr) :: x, y
+        integer(c_int) :: motion
+        real(c_double) :: motion_amp, motion_omega
+        integer(c_int) :: pcg_rel
+        real(c_double) :: pcg_tol
+        integer(c_int) :: pcg_maxit, gcl, precision, warm_start, patch_elems, patch_block
+    end type cfd_case_desc
+
+    ! one case of a batch (cfd_batch_case)
+    type, bind(c) :: cfd_batch_case
+        integer(c_int) :: nelem, npoin
+        type(c_ptr) :: inpoel, x, y
+        type(cfd_consts) :: c
+        type(cfd_bc_lists) :: bc
+    end type cfd_batch_case
+
+    ! field ids (CFD_F_*) and scalar slots (CFD_S_*, 0-based) of the header
+    integer(c_int), parameter :: cfd_f_u = 0, cfd_f_u1 = 1, cfd_f_rhs = 2, cfd_f_t = 3, &
+        cfd_f_vel_x = 4, cfd_f_vel_y = 5, cfd_f_w_x = 6, cfd_f_w_y = 7, cfd_f_x = 8, &
+        cfd_f_y = 9, cfd_f_x1 = 10, cfd_f_y1 = 11, cfd_f_rho = 12, cfd_f_e = 13, cfd_f_p = 14, &
+        cfd_f_rmach = 15, cfd_f_m = 16, cfd_f_gamm = 17, cfd_f_xpos = 18, cfd_f_ypos = 19, &
+        cfd_f_dtl = 20, cfd_f_dt = 21, cfd_f_shoc = 22, cfd_f_t_sugn1 = 23, &
+        cfd_f_t_sugn2 = 24, cfd_f_t_sugn3 = 25, cfd_f_area = 26, cfd_f_dnx = 27, &
+        cfd_f_dny = 28, cfd_f_hh = 29, cfd_f_hhx = 30, cfd_f_hhy = 31, cfd_f_lap_sparse = 32, &
+        cfd_f_lap_diag = 33
+    integer(c_int), parameter :: cfd_s_dtmin = 0, cfd_s_time = 1, cfd_s_dtmin1 = 2, &
+        cfd_s_bandera = 3, cfd_s_iter = 4, cfd_s_hmin = 5, cfd_s_fx = 6, cfd_s_fy = 7, &
+        cfd_s_rm = 8, cfd_s_disn2 = 9, cfd_s_pcg_itx = 10, cfd_s_pcg_ity = 11, &
+        cfd_s_pcg_rzx = 12, cfd_s_pcg_rzy = 13, cfd_s_ws_learned = 14, cfd_s_ws_solves = 15, &
+        cfd_s_ws_loss = 16, cfd_s_pcg_itsum = 17, cfd_nscalars = 18
 
     ! RK's module arrays (subrutinas.f90:645-849), nodal and per element
     type, bind(c) :: cfd_rk_nodes
@@ -196,6 +238,98 @@ module cfd_abi
             real(c_double), intent(in) :: w_x(*), w_y(*), dnx(*), dny(*), area(*)
             real(c_double), value :: dt
             integer(c_int), value :: mode
+        end function
+
+        ! ---- device-resident session (ns2DComp.ALE.f90:138-281) ----
+        integer(c_int) function cfd_session_create(s, m, d) bind(c, name='cfd_session_create')
+            import :: c_int, c_ptr, cfd_case_desc
+            type(c_ptr), intent(out) :: s
+            type(c_ptr), value :: m
+            type(cfd_case_desc), intent(in) :: d
+        end function
+        integer(c_int) function cfd_session_destroy(s) bind(c, name='cfd_session_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: s
+        end function
+        integer(c_int) function cfd_session_init_freestream(s) &
+            bind(c, name='cfd_session_init_freestream')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: s
+        end function
+        integer(c_int) function cfd_session_step(s, nsteps) bind(c, name='cfd_session_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: s
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function cfd_session_sync(s) bind(c, name='cfd_session_sync')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: s
+        end function
+        integer(c_int) function cfd_session_field_size(s, field, count) &
+            bind(c, name='cfd_session_field_size')
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: s
+            integer(c_int), value :: field
+            integer(c_long_long), intent(out) :: count
+        end function
+        integer(c_int) function cfd_session_get_field(s, field, host) &
+            bind(c, name='cfd_session_get_field')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: s
+            integer(c_int), value :: field
+            real(c_double), intent(out) :: host(*)
+        end function
+        integer(c_int) function cfd_session_set_field(s, field, host) &
+            bind(c, name='cfd_session_set_field')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: s
+            integer(c_int), value :: field
+            real(c_double), intent(in) :: host(*)
+        end function
+        integer(c_int) function cfd_session_get_scalars(s, scalars) &
+            bind(c, name='cfd_session_get_scalars')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: s
+            real(c_double), intent(out) :: scalars(*)
+        end function
+        integer(c_int) function cfd_session_set_scalars(s, scalars) &
+            bind(c, name='cfd_session_set_scalars')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: s
+            real(c_double), intent(in) :: scalars(*)
+        end function
+
+        ! ---- batches of independent fixed-mesh cases ----
+        integer(c_int) function cfd_batch_create(b, ncases, cases, precision, itlocal, &
+                                                 patch_elems, patch_block) &
+            bind(c, name='cfd_batch_create')
+            import :: c_int, c_ptr, cfd_batch_case
+            type(c_ptr), intent(out) :: b
+            integer(c_int), value :: ncases, precision, itlocal, patch_elems, patch_block
+            type(cfd_batch_case), intent(in) :: cases(*)
+        end function
+        integer(c_int) function cfd_batch_destroy(b) bind(c, name='cfd_batch_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+        end function
+        integer(c_int) function cfd_batch_step(b, nsteps) bind(c, name='cfd_batch_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function cfd_batch_get_field(b, icase, field, host) &
+            bind(c, name='cfd_batch_get_field')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: b
+            integer(c_int), value :: icase, field
+            real(c_double), intent(out) :: host(*)
+        end function
+        integer(c_int) function cfd_batch_get_scalars(b, icase, scalars) &
+            bind(c, name='cfd_batch_get_scalars')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: b
+            integer(c_int), value :: icase
+            real(c_double), intent(out) :: scalars(*)
         end function
     end interface
 

@@ -105,3 +105,74 @@ def test_fortran_driver_through_the_c_abi(tmp_path):
                     np.zeros(npn), xfix, fidx)
     assert iters == it_py and iters > 5
     assert np.array_equal(dbl["x"].view(np.uint64), x.view(np.uint64))
+
+
+def test_fortran_ids_match_the_header():
+    """The field ids and scalar slots the Fortran module declares are the
+    header's enums."""
+    import re
+    from cfd_amd import _lib
+    src = open(os.path.join(ROOT, "cfd_amd", "fortran", "cfd_abi.f90")).read()
+    ids = {k.upper(): int(v) for k, v in re.findall(r"\b(cfd_[fs]_\w+|cfd_nscalars)\s*=\s*(\d+)", src)}
+    fields = {k: v for k, v in _lib.FIELDS.items() if k != "CFD_NFIELDS"}
+    scalars = {k: v for k, v in _lib.SCALARS.items() if k != "CFD_NSCALARS"}
+    assert {k: ids[k] for k in fields} == fields
+    assert {k: ids[k] for k in scalars} == scalars
+    assert ids["CFD_NSCALARS"] == _lib.SCALARS["CFD_NSCALARS"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fortran_session_driver(tmp_path, precision):
+    """A Fortran program runs the device-resident session (create, free
+    stream, steps, fields, scalars) through the interface module; its state
+    equals the Python binding's session bit for bit."""
+    from cfd_amd.api import consts_of
+    from cfd_amd.build import SESSION_DEMO
+    from cfd_amd.mesh import acoustic_fsafe, cylinder_case
+    from cfd_amd.solver import Session
+
+    if not os.path.isfile(SESSION_DEMO):
+        pytest.skip("Fortran demo not built (flang absent at build time)")
+    cs = cylinder_case(24, 48, mach=0.5, re=300.0, alpha_deg=2.0, fsafe=acoustic_fsafe(0.5))
+    m, L = cs.mesh, cs.lists
+    nsteps = 10
+    c = consts_of(cs)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    i32 = lambda a: np.ascontiguousarray(a, np.int32).tofile(f)
+    f64 = lambda a: np.ascontiguousarray(a, np.float64).tofile(f)
+    with open(fin, "wb") as f:
+        i32([m.npoin, m.nelem, nsteps, precision])
+        f64([getattr(c, n) for n, _ in c._fields_])
+        i32(m.inpoel)
+        f64(m.x)
+        f64(m.y)
+        for ids, vals in (("ifixv_node", ("rfixv_valuex", "rfixv_valuey")),
+                          ("ifixrho_node", ("rfixrho_value",)), ("ifixt_node", ("rfixt_value",))):
+            i32([L[ids].size])
+            i32(L[ids])
+            for v in vals:
+                f64(L[v])
+        i32([L["iset"].shape[0]])
+        i32(L["iset"])
+        i32([L["nmove"], L["nfix_move"]])
+        i32(L["ilaux"])
+        f64([cs.xref, cs.yref])
+    print(f"running {SESSION_DEMO}, precision {precision}", flush=True)
+    r = subprocess.run([SESSION_DEMO, str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = np.fromfile(fout, np.float64)
+    U = raw[:4 * m.npoin].reshape(-1, 4)
+    T = raw[4 * m.npoin:5 * m.npoin]
+    sc = raw[5 * m.npoin:]
+    S = Session(cs, precision=precision)
+    S.step(nsteps)
+    assert np.isfinite(U).all()
+    assert np.array_equal(U, S.get("U")) and np.array_equal(T, S.get("T"))
+    ref = S.scalars()
+    from cfd_amd._lib import SCALARS
+    for k in ("DTMIN", "TIME", "ITER", "FX1", "FY1"):
+        assert sc[SCALARS["CFD_S_" + {"FX1": "FX", "FY1": "FY"}.get(k, k)]] == ref[k], k
+    S.close()
