@@ -209,12 +209,16 @@ k_forces(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
         }
         __syncthreads();
         if (lane == 0 && wave < 3) {
+            // unrolled so the LDS reads run ahead of the (sequential) sums
             const int m = nedge - base < 1024 ? nedge - base : 1024;
             if (wave == 0) {
+#pragma unroll 16
                 for (int q = 0; q < m; ++q) acc = acc + dfx[q];
             } else if (wave == 1) {
+#pragma unroll 16
                 for (int q = 0; q < m; ++q) acc = acc + dfy[q];
             } else {  // RM = RM + DFY*(XC-XR) - DFX*(YC-YR)
+#pragma unroll 16
                 for (int q = 0; q < m; ++q) acc = acc + m1[q] - m2[q];
             }
         }

@@ -300,6 +300,7 @@ __global__ void k_mlp_reduce(const float* __restrict__ gpart, const float* __res
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < kMlpParams) {
         float s = 0.f;
+#pragma unroll 8
         for (int b = 0; b < nblocks; ++b) s += gpart[(size_t)b * kMlpParams + p];
         grad[p] = s;
     }

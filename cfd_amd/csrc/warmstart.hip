@@ -214,7 +214,7 @@ void WarmStart::after_solves(const WsSolve& q, hipStream_t s) {
         k_ws_target<<<g, kBlock, 0, s>>>(q.XPOS, q.YPOS, ex.p, ey.p, wsc.p, tgt.p, np);
         // one Adam step on an evenly strided sample of about kWsTrainNodes nodes
         const int stride = std::max(1, np / kWsTrainNodes);
-        const int ns = mlp_gradient(mlp, feat.p, tgt.p, np, 16, s, stride);
+        const int ns = mlp_gradient(mlp, feat.p, tgt.p, np, 2, s, stride);
         mlp_adam(mlp, 1.0e-3f, s);
         k_copy_loss<<<1, 1, 0, s>>>(mlp.loss.p, 1.0 / ns, q.loss);
     }
