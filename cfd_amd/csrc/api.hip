@@ -589,6 +589,11 @@ struct cfd_session {
     // fixed value are zero): XPOS/YPOS stay zero and X, Y, W do not change
     // once one X += 0, W = 0/DTMIN pass has normalised signed zeros.
     bool xpos_zero = true, move_normalized = false;
+    // Under the reference's absolute PCG rule most warm-started solves stop at
+    // their first residual test; after a step whose two solves both did, the
+    // next step skips the learned candidate (and its training), as nothing
+    // can be gained.  The relative rule always uses it.
+    bool last_solves_trivial = false;
     // timing
     bool timing = false;
     unsigned tmask = 0;
@@ -704,6 +709,7 @@ struct cfd_session {
         const double tol = d.pcg_tol > 0.0 ? d.pcg_tol : 1.0e-10;
         const int maxit = d.pcg_maxit > 0 ? d.pcg_maxit : 1000;
         WsSolve q;
+        const bool ws_now = ws_on && !(d.pcg_rel == 0 && last_solves_trivial);
         if (ws_on) {
             q.XPOS = XPOS.p;
             q.YPOS = YPOS.p;
@@ -726,12 +732,13 @@ struct cfd_session {
             q.n_learned = scal(CFD_S_WS_LEARNED);
             q.n_solves = scal(CFD_S_WS_SOLVES);
             q.loss = scal(CFD_S_WS_LOSS);
-            ws.before_solves(q, s);
+            if (ws_now) ws.before_solves(q, s);  // else the reference's guess
         }
         const PcgSystem sys[2] = {{&pcg, XPOS.p, BZERO.p, POSAUX.p, ILAUX.p, nn},
                                   {&pcg_y, YPOS.p, BZERO.p, POSAUY.p, ILAUX.p, nn}};
         pcg_launch_multi(sys, 2, np, LAP.p, m->t.idx.p, m->t.rowptr.p, LDIAG.p, tol, maxit,
                          d.pcg_rel, s);
+        last_solves_trivial = pcg_polled_iterations(pcg) == 0.0 && pcg_polled_iterations(pcg_y) == 0.0;
         te(CFD_K_PCG);
         pcg_store_result(pcg, scal(CFD_S_PCG_ITX), scal(CFD_S_PCG_RZX), scal(CFD_S_PCG_ITSUM), s);
         pcg_store_result(pcg_y, scal(CFD_S_PCG_ITY), scal(CFD_S_PCG_RZY), scal(CFD_S_PCG_ITSUM), s);
@@ -1001,6 +1008,9 @@ int cfd_session_init_freestream(cfd_session* S) {
         S->put_scalars();
         S->w_dirty = false;
         S->move_normalized = false;
+        S->last_solves_trivial = false;
+        S->ws.nhist = 0;  // the learned start's history restarts with the run
+        S->ws.predicted = false;
     })
 }
 

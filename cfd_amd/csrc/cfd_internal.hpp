@@ -451,6 +451,9 @@ struct PcgSystem {
 void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A, const int* idx,
                       const int* rowptr, const double* diag, double tol, int maxit, int rel,
                       hipStream_t s);
+// iterations of the last solve run by pcg_launch_multi on w, as its final
+// poll read them (host memory, no device access)
+double pcg_polled_iterations(const PcgWork& w);
 struct PcgResult {
     int iters;
     double rz;

@@ -389,6 +389,8 @@ void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A
     }
 }
 
+double pcg_polled_iterations(const PcgWork& w) { return w.hpin.p ? w.hpin.p[S_K] : -1.0; }
+
 void pcg_launch(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,
                 const double* diag, double* x, const double* b, const double* x_fix,
                 const int* x_fix_idx, int nfix, double tol, int maxit, int rel, hipStream_t s) {

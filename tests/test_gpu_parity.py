@@ -519,3 +519,27 @@ def test_learned_warm_start_cuts_mesh_solve_iterations():
     print("learned diff ", ["%.1e" % v for v in diff[4::5]])
     assert floor[nsteps // 3:].max() > 1e-4           # 1e-4 force agreement is below round-off chaos
     assert diff.max() < 0.1                             # the same flow
+
+
+def test_learned_warm_start_gate_under_the_absolute_rule():
+    """Under the reference's absolute PCG rule most solves stop at their first
+    test: after a step whose two solves both did, the learned candidate is
+    skipped (the reference's guess is used), so fewer solves see a candidate
+    than with the relative rule, and the run equals the reference-guess run's
+    mesh to the rule's accuracy."""
+    from cfd_amd.solver import Session
+    cs = ellipse_case(48, 96)
+    nsteps = 20
+    out = {}
+    for ws in (0, 1):
+        S = Session(cs, pcg_rel=0, warm_start=ws)
+        S.step(nsteps)
+        out[ws] = dict(sc=S.scalars(), X=S.get("X"), Y=S.get("Y"))
+        S.close()
+    p, q = out[0]["sc"], out[1]["sc"]
+    print(f"absolute rule: iterations {p['PCG_ITSUM']:.0f} / {q['PCG_ITSUM']:.0f}, candidates on "
+          f"{q['WS_SOLVES']:.0f} of {2 * (nsteps - 2)} solves")
+    assert q["WS_SOLVES"] < 2 * (nsteps - 2)
+    assert np.isfinite(out[1]["X"]).all()
+    disp = np.abs(out[0]["X"] - cs.mesh.x).max()
+    assert np.abs(out[1]["X"] - out[0]["X"]).max() <= 1e-2 * disp
