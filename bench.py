@@ -180,6 +180,51 @@ def batch_sweep(steps: int = 100, warmup: int = 5, ncases: int = 256) -> dict:
             "states_finite": finite}
 
 
+def large_ale(steps: int = 3, warmup: int = 2, tol: float = 1e-8, maxit: int = 50000) -> dict:
+    """The large ALE run: pitching ellipse O-grid 4096 x 4096 (16,777,216 nodes,
+    33,546,240 triangles), M=0.7, Re=10000, the GCL volume update on; plain
+    (fused FP64, previous-step guess) against learned (fused FP32 element,
+    learned warm start) at the relative PCG rule `tol` with `maxit`
+    iterations allowed (the reference's 1000 end every solve unconverged at
+    this size), a few steps each (the mesh solves dominate; at this size one
+    plain step takes seconds)."""
+    from cfd_amd.mesh import ellipse_case
+    from cfd_amd.solver import Session
+    cs = ellipse_case(4096, 4096, mach=0.7, re=10000.0)
+    runs = {}
+    for name, prec, ws in (("plain_fp64", 0, 0), ("learned_fp32", 1, 1)):
+        t0 = time.perf_counter()
+        S = Session(cs, precision=prec, pcg_rel=1, pcg_tol=tol, pcg_maxit=maxit, warm_start=ws,
+                    gcl=1)
+        t_create = time.perf_counter() - t0
+        print(f"large_ale {name}: created in {t_create:.1f} s", file=sys.stderr, flush=True)
+        for _ in range(warmup):
+            S.step(1)
+            S.sync()
+            print(f"large_ale {name}: warm-up step, PCG iterations so far "
+                  f"{S.scalars()['PCG_ITSUM']:.0f}", file=sys.stderr, flush=True)
+        it0 = S.scalars()["PCG_ITSUM"]
+        sec = 0.0
+        for _ in range(steps):  # one step at a time: progress lines between
+            t0 = time.perf_counter()
+            S.step(1)
+            S.sync()
+            sec += time.perf_counter() - t0
+            print(f"large_ale {name}: step, {sec:.1f} s so far", file=sys.stderr, flush=True)
+        sc = S.scalars()
+        runs[name] = {"ms_per_step": sec / steps * 1e3, "create_s": t_create,
+                      "pcg_iterations_per_step": (sc["PCG_ITSUM"] - it0) / steps,
+                      "drag_fx": sc["FX1"], "lift_fy": sc["FY1"]}
+        if ws:
+            runs[name]["learned_guess_used"] = f"{sc['WS_LEARNED']:.0f}/{sc['WS_SOLVES']:.0f}"
+        S.close()
+    runs["speedup"] = runs["plain_fp64"]["ms_per_step"] / runs["learned_fp32"]["ms_per_step"]
+    runs["workload"] = (f"ellipse O-grid 4096x4096 ({cs.npoin} nodes, {cs.nelem} triangles), M=0.7, "
+                        f"Re=10000, pitching, GCL on, relative PCG rule {tol:g} (at most {maxit} "
+                        f"iterations), {warmup} untimed + {steps} timed steps")
+    return runs
+
+
 def read_traffic(name: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), committed under profiles/; None when absent."""
@@ -243,6 +288,8 @@ def main():
     ap.add_argument("--ale-steps", type=int, default=40)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--large-ale", action="store_true",
+                    help="also run the 4096 x 4096 pitching ellipse (minutes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the path is a single-GPU kernel set "
@@ -303,6 +350,8 @@ def main():
         out["ale"] = ale_compare(args.ale_steps)
     if not args.no_batch:
         out["batch"] = batch_sweep(args.batch_steps)
+    if args.large_ale:
+        out["large_ale"] = large_ale()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cs)
     print(json.dumps(out))
