@@ -39,6 +39,17 @@ def ell():
 
 
 @pytest.fixture(scope="module")
+def slip():
+    # body edges as slip walls: NORMALES + NORMALVEL on every stage
+    from cfd_amd.mesh import acoustic_fsafe
+    cs = cylinder_case(40, 80, mach=0.5, re=500.0, slip_body=True, fsafe=acoustic_fsafe(0.5))
+    o = OracleSolver(cs)
+    for _ in range(15):
+        o.step()
+    return cs, o
+
+
+@pytest.fixture(scope="module")
 def mctx(cyl):
     from cfd_amd.api import MeshContext
     cs, _ = cyl
@@ -338,7 +349,7 @@ def _session_from_oracle(cs, o, precision=0):
 
 
 @pytest.mark.parametrize("precision", [0, 2])
-@pytest.mark.parametrize("which", ["cyl", "ell"])
+@pytest.mark.parametrize("which", ["cyl", "ell", "slip"])
 def test_session_teacher_forced(which, precision, request):
     """Each step from the oracle's state; the GPU step must land within 1e-12:
     precision 0 = the fused FP64 path (regrouped element arithmetic, patch
