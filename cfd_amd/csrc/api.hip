@@ -87,23 +87,7 @@ struct cfd_mesh {
     void sync() { CFD_HIP(hipStreamSynchronize(s)); }
 };
 
-#define CFD_TRY(...)                                    \
-    try {                                               \
-        __VA_ARGS__;                                    \
-        return CFD_OK;                                  \
-    } catch (const cfd::Error& e) {                     \
-        set_last_error(e.what());                       \
-        std::string w = e.what();                       \
-        if (w.find("mesh:") == 0) return CFD_ERR_MESH;  \
-        if (w.find("state:") == 0) return CFD_ERR_STATE;\
-        if (w.find("arg:") == 0) return CFD_ERR_ARG;    \
-        return CFD_ERR_DEVICE;                          \
-    } catch (const std::exception& e) {                 \
-        set_last_error(e.what());                       \
-        return CFD_ERR_DEVICE;                          \
-    }
 
-#define CFD_ARG(cond, msg) CFD_CHECK(cond, std::string("arg: ") + msg)
 
 extern "C" {
 
@@ -949,6 +933,9 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
             CFD_ARG(pe <= 682, "patch_elems must be at most 682");
             CFD_ARG(blk % 64 == 0 && blk <= 512, "patch_block must be a multiple of 64 up to 512");
             build_patches(S->patches, inp.data(), d->X, d->Y, ne, np, pe, blk, s);
+            CFD_ARG(patch_lds_fits(S->patches.max_nodes, pe,
+                                   d->precision == 1 ? sizeof(float4) : sizeof(double4)),
+                    "patch_elems too large: a patch exceeds the fused kernel's 64 KB of LDS");
         } else {
             S->c_calc.alloc(12 * (size_t)ne);
             S->c_fuente.alloc(12 * (size_t)ne);

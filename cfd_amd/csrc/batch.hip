@@ -231,22 +231,7 @@ struct cfd_batch {
     }
 };
 
-#define CFD_TRY(...)                                    \
-    try {                                               \
-        __VA_ARGS__;                                    \
-        return CFD_OK;                                  \
-    } catch (const cfd::Error& e) {                     \
-        set_last_error(e.what());                       \
-        std::string w = e.what();                       \
-        if (w.find("mesh:") == 0) return CFD_ERR_MESH;  \
-        if (w.find("arg:") == 0) return CFD_ERR_ARG;    \
-        return CFD_ERR_DEVICE;                          \
-    } catch (const std::exception& e) {                 \
-        set_last_error(e.what());                       \
-        return CFD_ERR_DEVICE;                          \
-    }
 
-#define CFD_ARG(cond, msg) CFD_CHECK(cond, std::string("arg: ") + msg)
 
 
 extern "C" {
@@ -394,6 +379,9 @@ int cfd_batch_create(cfd_batch** out, int ncases, const cfd_batch_case* cases, i
         CFD_ARG(blk % 64 == 0 && blk >= 64 && blk <= 512,
                 "patch_block must be a multiple of 64 from 64 to 512");
         B->build(pe, blk);
+        CFD_ARG(patch_lds_fits(B->patches.max_nodes, pe,
+                               precision == 1 ? sizeof(float4) : sizeof(double4)),
+                "patch_elems too large: a patch exceeds the fused kernel's 64 KB of LDS");
         B->init_freestream();
         CFD_HIP(hipStreamSynchronize(s));
         *out = B.release();

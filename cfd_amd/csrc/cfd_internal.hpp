@@ -45,6 +45,27 @@ void set_last_error(const std::string& msg);
         if (!(cond)) throw ::cfd::Error(msg);                                          \
     } while (0)
 
+// C-ABI entry bodies: a cfd::Error becomes a status code by its message's
+// prefix ("mesh:", "state:", "arg:", anything else a device error), any
+// other exception a device error; the message is kept for cfd_last_error().
+#define CFD_TRY(...)                                    \
+    try {                                               \
+        __VA_ARGS__;                                    \
+        return CFD_OK;                                  \
+    } catch (const cfd::Error& e) {                     \
+        set_last_error(e.what());                       \
+        std::string w = e.what();                       \
+        if (w.find("mesh:") == 0) return CFD_ERR_MESH;  \
+        if (w.find("state:") == 0) return CFD_ERR_STATE;\
+        if (w.find("arg:") == 0) return CFD_ERR_ARG;    \
+        return CFD_ERR_DEVICE;                          \
+    } catch (const std::exception& e) {                 \
+        set_last_error(e.what());                       \
+        return CFD_ERR_DEVICE;                          \
+    }
+
+#define CFD_ARG(cond, msg) CFD_CHECK(cond, std::string("arg: ") + msg)
+
 // Owned device allocation.
 template <class T>
 struct DBuf {

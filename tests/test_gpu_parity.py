@@ -556,3 +556,20 @@ def test_learned_warm_start_gate_under_the_absolute_rule():
     # displacement of 2e-4 (measured 1.0e-2 of it)
     disp = np.abs(out[0]["X"] - cs.mesh.x).max()
     assert np.abs(out[1]["X"] - out[0]["X"]).max() <= 1e-1 * disp
+
+
+def test_session_rejects_patch_shapes_beyond_lds():
+    """A patch shape whose fused-kernel LDS exceeds 64 KB (FP64: 682
+    elements need 65 KB of contribution slots alone) is refused when the
+    session is created; bad thread counts too."""
+    from cfd_amd._lib import CfdError
+    from cfd_amd.solver import Session
+    cs = cylinder_case(24, 48)
+    with pytest.raises(CfdError):
+        Session(cs, precision=0, patch_elems=682)
+    with pytest.raises(CfdError):
+        Session(cs, precision=1, patch_block=96)
+    S = Session(cs, precision=1, patch_elems=682, patch_block=512)   # fits in binary32
+    S.step(2)
+    assert np.isfinite(S.get("U")).all()
+    S.close()
