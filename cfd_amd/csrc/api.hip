@@ -660,7 +660,7 @@ struct cfd_session {
         o.fac = fac;
         o.area = AREA.p;
         o.dtl = DTL.p;
-        o.perm = patches.perm.p;
+        o.inv_perm = patches.inv_perm.p;
         o.cte = c.cte;
         return o;
     }

@@ -82,7 +82,7 @@ struct cfd_batch {
         o.fac = fac;
         o.area = AREA.p;
         o.dtl = DTL.p;
-        o.perm = patches.perm.p;
+        o.inv_perm = patches.inv_perm.p;
         o.cte = c.cte;
         o.ecase = ecase.p;
         o.cases = cases.p;

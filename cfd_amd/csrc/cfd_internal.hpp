@@ -186,12 +186,13 @@ void launch_deltat_chunks(const Tables& t, const Consts& c, const double* area, 
 // Inputs of the patch element residual (nelem values each, patch order, of
 // the element type R: float on the FP32 path, double on the fused FP64
 // path), written by ESTAB in place of its element-order outputs: nu =
-// SHOC*CTE, tau1..3, fac = area*dtl/3; position q holds element perm[q].
+// SHOC*CTE, tau1..3, fac = area*dtl/3; element e's values sit at patch
+// position inv_perm[e].
 template <class R>
 struct ElemP {
     R *nu = nullptr, *ta = nullptr, *tb = nullptr, *tc = nullptr, *fac = nullptr;
     const double *area = nullptr, *dtl = nullptr;
-    const int* perm = nullptr;
+    const int* inv_perm = nullptr;  // element e's patch position
     double cte = 1.0;
     // batches: element e belongs to case ecase[e]; its DTMIN is
     // dtmin[ecase[e] * dt_stride] and its constants cases[ecase[e]]

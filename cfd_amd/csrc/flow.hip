@@ -83,8 +83,9 @@ __global__ void k_deltat_clip(double* __restrict__ DT, const double* __restrict_
 
 // ---------------------------------------------------------------------------
 // ESTAB.  kPatch = false: the FP64 outputs SHOC, T_SUGN1..3 per element.
-// kPatch = true (fused paths): thread q handles element perm[q] and writes
-// only the inputs of the fused residual, rounded to R, at q (patch order).
+// kPatch = true (fused paths): writes only the inputs of the fused residual,
+// rounded to R, at the element's patch position (the reads stay in element
+// order and coalesced; the writes scatter).
 // ---------------------------------------------------------------------------
 template <class R, bool kPatch>
 __global__ void __launch_bounds__(kBlock)
@@ -95,9 +96,9 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
         const double* __restrict__ dNy, int nelem, double fr_, double tinf_, double rhoinf_,
         const double* __restrict__ dtminp, double* __restrict__ shoc, double* __restrict__ t1,
         double* __restrict__ t2, double* __restrict__ t3, ElemP<R> o) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nelem) return;
-    const int e = kPatch ? o.perm[q] : q;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    const int q = kPatch ? o.inv_perm[e] : e;
     const int cs = (kPatch && o.ecase) ? o.ecase[e] : 0;
     const double dtmin = dtminp[(size_t)cs * o.dt_stride];
     const Consts* cq = (kPatch && o.cases) ? o.cases + cs : nullptr;
@@ -891,7 +892,7 @@ void launch_estab_patch(const Tables& t, const Consts& c, const double* U, const
                         const double* GAMM, const double* VX, const double* VY, const double* WX,
                         const double* WY, const double* dNx, const double* dNy,
                         const double* dtmin, const ElemP<R>& o, hipStream_t s) {
-    CFD_CHECK(o.nu && o.perm, "internal: patch outputs missing");
+    CFD_CHECK(o.nu && o.inv_perm, "internal: patch outputs missing");
     k_estab<R, true><<<grid_for(t.nelem), kBlock, 0, s>>>(
         t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy, t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin,
         nullptr, nullptr, nullptr, nullptr, o);
