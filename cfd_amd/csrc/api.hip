@@ -882,7 +882,7 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         CFD_ARG(d->nsetedges == 0 || d->iset, "iset");
         CFD_ARG(d->precision >= 0 && d->precision <= 2,
                 "precision must be 0 (FP64), 1 (FP32 element) or 2 (FP64, reference order)");
-        CFD_ARG(d->warm_start == 0 || d->warm_start == 1, "warm_start must be 0 or 1");
+        CFD_ARG(d->warm_start >= 0 && d->warm_start <= 2, "warm_start must be 0, 1 or 2");
         auto S = std::make_unique<cfd_session>();
         S->m = m;
         S->s = m->s;
@@ -963,12 +963,13 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->bc.upload(h, np, &d->bc, s);
         S->pcg.ensure(np);
         S->pcg_y.ensure(np);
-        S->ws_on = d->warm_start == 1 && d->nmove > 0;
+        S->ws_on = d->warm_start != 0 && d->nmove > 0;
         if (S->ws_on) {  // length scale: farthest node from the pivot
             double l = 0.0;
             for (int i = 0; i < np; ++i)
                 l = std::max(l, std::hypot(d->X[i] - d->xref, d->Y[i] - d->yref));
             S->ws.init(np, l, s);
+            S->ws.use_mlp = d->warm_start == 1;
         }
         S->sc[CFD_S_BANDERA] = 1.0;
         S->put_scalars();
@@ -997,7 +998,7 @@ int cfd_session_init_freestream(cfd_session* S) {
         S->move_normalized = false;
         S->last_solves_trivial = false;
         S->ws.nhist = 0;  // the learned start's history restarts with the run
-        S->ws.predicted = false;
+        S->ws.predicted = S->ws.net_used = false;
     })
 }
 

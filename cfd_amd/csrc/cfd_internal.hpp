@@ -365,9 +365,11 @@ struct WsSolve {
 struct WarmStart {
     int np = 0, nhist = 0;
     bool predicted = false;
+    bool use_mlp = true;  // false: the extrapolation alone (the network's ablation)
+    bool net_used = false;  // this step's candidate includes the network's correction
     double inv_l = 1.0;
     MlpWeights mlp;
-    DBuf<double> h1x, h2x, h1y, h2y, ex, ey, hatx, haty, wsc, partials;
+    DBuf<double> h1x, h2x, h3x, h1y, h2y, h3y, ex, ey, hatx, haty, wsc, partials;
     DBuf<unsigned short> feat;
     DBuf<float> out, tgt;
     void init(int npoin, double length, hipStream_t s);

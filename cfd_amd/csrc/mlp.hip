@@ -294,20 +294,35 @@ __global__ void __launch_bounds__(64) k_mlp_grad(MlpDev m, const unsigned short*
     if (lane == 0) lpart[blockIdx.x] = loss;
 }
 
-// grad[p] = sum over blocks of gpart[b][p] in block order; loss likewise
-__global__ void k_mlp_reduce(const float* __restrict__ gpart, const float* __restrict__ lpart,
-                             int nblocks, float* __restrict__ grad, float* __restrict__ loss) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// grad[p] = sum over blocks of gpart[b][p]; the loss likewise from lpart (as
+// column kMlpParams).  A block sums 64 columns; each of its kRedWaves waves
+// sums one contiguous range of the partial blocks, and the waves' sums are
+// added in wave order: a fixed order for a given block count.
+constexpr int kRedWaves = 16;
+__global__ void __launch_bounds__(64 * kRedWaves)
+k_mlp_reduce(const float* __restrict__ gpart, const float* __restrict__ lpart, int nblocks,
+             float* __restrict__ grad, float* __restrict__ loss) {
+    __shared__ float sm[kRedWaves][65];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    const int b0 = (int)((long long)nblocks * wv / kRedWaves);
+    const int b1 = (int)((long long)nblocks * (wv + 1) / kRedWaves);
+    float s = 0.f;
     if (p < kMlpParams) {
-        float s = 0.f;
 #pragma unroll 8
-        for (int b = 0; b < nblocks; ++b) s += gpart[(size_t)b * kMlpParams + p];
-        grad[p] = s;
+        for (int b = b0; b < b1; ++b) s += gpart[(size_t)b * kMlpParams + p];
+    } else if (p == kMlpParams) {
+#pragma unroll 8
+        for (int b = b0; b < b1; ++b) s += lpart[b];
     }
-    if (p == 0) {
-        float s = 0.f;
-        for (int b = 0; b < nblocks; ++b) s += lpart[b];
-        *loss = s;
+    sm[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0) {
+        float t = 0.f;
+#pragma unroll
+        for (int w = 0; w < kRedWaves; ++w) t += sm[w][lane];
+        if (p < kMlpParams) grad[p] = t;
+        else if (p == kMlpParams) *loss = t;
     }
 }
 
@@ -380,8 +395,8 @@ int mlp_gradient(MlpWeights& w, const unsigned short* X, const float* T, int n, 
     if ((size_t)nb > w.lpart.n) w.lpart.alloc(nb);
     k_mlp_grad<<<nb, 64, 0, s>>>(mlp_dev(w), X, T, ns, stride, tiles_per_block, 1.0f / (float)ns,
                                  w.gpart.p, w.lpart.p);
-    k_mlp_reduce<<<grid_for(kMlpParams), kBlock, 0, s>>>(w.gpart.p, w.lpart.p, nb, w.grad.p,
-                                                         w.loss.p);
+    k_mlp_reduce<<<kMlpParams / 64 + 1, 64 * kRedWaves, 0, s>>>(w.gpart.p, w.lpart.p, nb,
+                                                                  w.grad.p, w.loss.p);
     CFD_HIP(hipGetLastError());
     return ns;
 }

@@ -210,7 +210,9 @@ typedef struct {
                                    the last two solutions plus the MLP's
                                    correction, trained online on the solver's
                                    own solutions) when its initial residual is
-                                   smaller, else the previous solution       */
+                                   smaller, else the previous solution
+                                2: the same with the extrapolation alone (no
+                                   network: the ablation of 1)                */
     int patch_elems;         /* fused paths (precision 0, 1): elements per
                                 patch, 1..682 within the kernel's 64 KB of
                                 LDS (0: 512 FP32, 384 FP64)                   */

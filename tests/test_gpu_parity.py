@@ -481,9 +481,10 @@ def test_fp32_free_run_density_and_forces(cyl):
 
 
 def test_learned_warm_start_cuts_mesh_solve_iterations():
-    """Moving ellipse, relative PCG rule 1e-6: the learned warm start (MLP on
-    the matrix cores over the extrapolated solution history, with fallback)
-    needs far fewer mesh-solve iterations than the previous-step guess, starts
+    """Moving ellipse, relative PCG rule 1e-6: the learned warm start (the
+    solution history extrapolated, plus the MLP's correction on the matrix
+    cores, with fallback) needs far fewer mesh-solve iterations than the
+    previous-step guess, starts
     from the learned guess on most solves, and the mesh agrees with the
     reference-guess run to the solve tolerance.  The forces of this pitching
     flow are chaotic at round-off level (tests/test_oracle_chaos.py shows it
@@ -530,6 +531,37 @@ def test_learned_warm_start_cuts_mesh_solve_iterations():
     print("learned diff ", ["%.1e" % v for v in diff[4::5]])
     assert floor[nsteps // 3:].max() > 1e-4           # 1e-4 force agreement is below round-off chaos
     assert diff.max() < 0.1                             # the same flow
+
+
+@pytest.mark.parametrize("tol", [1e-7, 1e-8])
+def test_trained_network_gains_over_the_extrapolation_alone(tol):
+    """What the network adds: the same moving-ellipse run (FP32 elements)
+    with the learned start (extrapolation + the network's Galerkin-scaled
+    correction) and with the extrapolation alone (warm_start 2, no network),
+    both under the same relative PCG rule (measured ratios 0.73 and 0.76).  From the fourth step on (the first with the network)
+    the learned start needs clearly fewer mesh-solve iterations, and both
+    meshes agree with each other to the rule's accuracy."""
+    from cfd_amd.solver import Session
+    cs = ellipse_case(48, 96)
+    nsteps, n0 = 30, 3
+    out = {}
+    for ws in (2, 1):
+        S = Session(cs, precision=1, pcg_rel=1, pcg_tol=tol, warm_start=ws)
+        S.step(n0)
+        it0 = S.scalars()["PCG_ITSUM"]
+        S.step(nsteps - n0)
+        sc = S.scalars()
+        out[ws] = dict(its=sc["PCG_ITSUM"] - it0, sc=sc, X=S.get("X"), Y=S.get("Y"))
+        S.close()
+    e, q = out[2], out[1]
+    print(f"tol {tol:g}: iterations after step {n0}: extrapolation {e['its']:.0f}, "
+          f"learned {q['its']:.0f} (loss {q['sc']['WS_LOSS']:.3e})")
+    assert e["sc"]["WS_SOLVES"] == q["sc"]["WS_SOLVES"] == 2 * (nsteps - 2)
+    assert q["its"] <= 0.85 * e["its"]
+    assert np.isfinite(q["sc"]["WS_LOSS"]) and q["sc"]["WS_LOSS"] < 1.0
+    disp = np.abs(e["X"] - cs.mesh.x).max() + np.abs(e["Y"] - cs.mesh.y).max()
+    dx = max(np.abs(q["X"] - e["X"]).max(), np.abs(q["Y"] - e["Y"]).max())
+    assert disp > 0 and dx <= 1e-2 * disp
 
 
 def test_learned_warm_start_gate_under_the_absolute_rule():
