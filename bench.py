@@ -142,6 +142,8 @@ def ale_compare(steps: int = 40, warmup: int = 5) -> dict:
         runs["chaos_floor_drag"] = abs(u["drag_fx"] - p["drag_fx"]) / scale
         runs["chaos_floor_lift"] = abs(u["lift_fy"] - p["lift_fy"]) / scale
         res[rule] = runs
+        progress(f"ale {rule}: learned {q['ms_per_step']:.2f} ms/step, "
+                 f"extrap {runs['extrap_fp32']['ms_per_step']:.2f}, plain {p['ms_per_step']:.2f}")
     res["workload"] = (f"ellipse O-grid 512x1024 ({cs.npoin} nodes, {cs.nelem} triangles), M=0.5, "
                        f"Re=5000, pitching, {warmup} untimed + {steps} timed steps")
     return res
@@ -185,19 +187,20 @@ def batch_sweep(steps: int = 100, warmup: int = 5, ncases: int = 256) -> dict:
             "states_finite": finite}
 
 
-def large_ale(steps: int = 3, warmup: int = 2, tol: float = 1e-8, maxit: int = 50000) -> dict:
+def large_ale(steps: int = 3, warmup: int = 3, tol: float = 1e-8, maxit: int = 50000) -> dict:
     """The large ALE run: pitching ellipse O-grid 4096 x 4096 (16,777,216 nodes,
     33,546,240 triangles), M=0.7, Re=10000, the GCL volume update on; plain
     (fused FP64, previous-step guess) against learned (fused FP32 element,
-    learned warm start) at the relative PCG rule `tol` with `maxit`
-    iterations allowed (the reference's 1000 end every solve unconverged at
-    this size), a few steps each (the mesh solves dominate; at this size one
-    plain step takes seconds)."""
+    learned warm start) and extrap (the same without the network) at the
+    relative PCG rule `tol` with `maxit` iterations allowed (the reference's
+    1000 end every solve unconverged at this size), a few steps each (the
+    mesh solves dominate; at this size one plain step takes seconds).  The
+    three warm-up steps fill the three-solution history the network reads."""
     from cfd_amd.mesh import ellipse_case
     from cfd_amd.solver import Session
     cs = ellipse_case(4096, 4096, mach=0.7, re=10000.0)
     runs = {}
-    for name, prec, ws in (("plain_fp64", 0, 0), ("learned_fp32", 1, 1)):
+    for name, prec, ws in (("plain_fp64", 0, 0), ("extrap_fp32", 1, 2), ("learned_fp32", 1, 1)):
         t0 = time.perf_counter()
         S = Session(cs, precision=prec, pcg_rel=1, pcg_tol=tol, pcg_maxit=maxit, warm_start=ws,
                     gcl=1)
@@ -221,13 +224,19 @@ def large_ale(steps: int = 3, warmup: int = 2, tol: float = 1e-8, maxit: int = 5
                       "pcg_iterations_per_step": (sc["PCG_ITSUM"] - it0) / steps,
                       "drag_fx": sc["FX1"], "lift_fy": sc["FY1"]}
         if ws:
-            runs[name]["learned_guess_used"] = f"{sc['WS_LEARNED']:.0f}/{sc['WS_SOLVES']:.0f}"
+            runs[name]["predicted_guess_used"] = f"{sc['WS_LEARNED']:.0f}/{sc['WS_SOLVES']:.0f}"
         S.close()
     runs["speedup"] = runs["plain_fp64"]["ms_per_step"] / runs["learned_fp32"]["ms_per_step"]
+    runs["speedup_extrap"] = runs["plain_fp64"]["ms_per_step"] / runs["extrap_fp32"]["ms_per_step"]
     runs["workload"] = (f"ellipse O-grid 4096x4096 ({cs.npoin} nodes, {cs.nelem} triangles), M=0.7, "
                         f"Re=10000, pitching, GCL on, relative PCG rule {tol:g} (at most {maxit} "
                         f"iterations), {warmup} untimed + {steps} timed steps")
     return runs
+
+
+def progress(msg: str) -> None:
+    """One line per finished section on stderr (stdout carries the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def read_traffic(name: str):
@@ -341,6 +350,7 @@ def main():
         "kernels": r["kernels"],
         "state": r["state"],
     }
+    progress(f"main path: {r['ms_per_step']:.3f} ms/step")
     if not args.no_variants:
         # the other two element paths, shorter runs of the same workload
         out["variants"] = {}
@@ -351,14 +361,17 @@ def main():
                     "path": PATH[p], "value": v["value"], "ms_per_step": v["ms_per_step"],
                     "kernel": KERNEL[p], "avg_launch_us": v["avg_elem_us"],
                     "roofline_frac": v["achieved"] / HBM_PEAK_GBS, "steps": 20, "warmup": 20}
+                progress(f"variant {name}: {v['ms_per_step']:.3f} ms/step")
     if not args.no_ale:
         out["ale"] = ale_compare(args.ale_steps)
     if not args.no_batch:
         out["batch"] = batch_sweep(args.batch_steps)
+        progress("batch sweep done")
     if args.large_ale:
         out["large_ale"] = large_ale()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cs)
+        progress("cpu baseline done")
     print(json.dumps(out))
 
 

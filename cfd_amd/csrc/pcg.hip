@@ -23,6 +23,13 @@
 // partials in a fixed tree, so the iteration sequence is deterministic (not
 // the serial summation order of the reference).
 //
+// Two systems on one matrix (the x and y mesh solves) iterate in lockstep
+// through paired kernels: one launch per kernel for both, with the matrix
+// entries staged once for both products.  Each system's arithmetic, block
+// decomposition and partial order are those of the single-system kernels,
+// so the paired iterations are bit for bit the single ones; a system that
+// has finished is skipped inside the paired kernels.
+//
 // Relative mode (rel = 1): the two tests use tol^2 times the r.r and r.z of
 // the residual of the guess that is zero off the fixed nodes, so the stopping
 // point does not depend on the initial guess (warm starts compare at equal
@@ -266,6 +273,113 @@ k_fin_b(const double* __restrict__ partials, int nb, double* __restrict__ scal) 
     }
 }
 
+// ---- paired systems (0 and 1): the kernels above, both systems per launch
+struct PairSys {
+    double *x, *r, *z, *y, *partial, *scal;
+    const double *pcur, *pold;
+    double* pnew;
+    const int* fixflag;
+};
+
+__global__ void __launch_bounds__(kBlock)
+k_iter_a2(int n, PairSys s0, PairSys s1, const double* __restrict__ diag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const PairSys& q = k ? s1 : s0;
+        if (q.scal[S_DONE] != 0.0) continue;  // uniform over the grid
+        const double alfa = q.scal[S_ALFA];
+        if (q.scal[S_STOP] != 0.0) {
+            if (i < n) q.x[i] = alfa * q.pcur[i] + q.x[i];
+            continue;
+        }
+        double v = 0.0;
+        if (i < n) {
+            q.x[i] = alfa * q.pcur[i] + q.x[i];
+            double ri = -alfa * q.y[i] + q.r[i];
+            q.r[i] = ri;
+            double zi = ri / diag[i];
+            q.z[i] = zi;
+            v = ri * zi;
+        }
+        block_partial(v, OpSum(), q.partial);
+    }
+}
+
+// block k finalizes system k (k_fin_a's body)
+__global__ void __launch_bounds__(256)
+k_fin_a2(PairSys s0, PairSys s1, int nb) {
+    const PairSys& q = blockIdx.x ? s1 : s0;
+    double* scal = q.scal;
+    if (scal[S_DONE] != 0.0) return;
+    if (scal[S_STOP] != 0.0) {
+        if (threadIdx.x == 0) {
+            scal[S_DONE] = 1.0;
+            scal[S_CONV] = fabs(scal[S_EOLD]) > scal[S_TOLZ] ? 0.0 : 1.0;
+        }
+        return;
+    }
+    const double e = combine_partials(q.partial, nb, 0.0, OpSum());
+    if (threadIdx.x == 0) {
+        scal[S_K] = scal[S_K] + 1.0;
+        scal[S_ENEW] = e;
+        scal[S_BETA] = e / scal[S_EOLD];
+    }
+}
+
+// k_iter_b (not the first iteration) for both systems, A staged once
+__global__ void __launch_bounds__(kBlock)
+k_iter_b2(int n, const double* __restrict__ A, const int* __restrict__ idx,
+          const int* __restrict__ rowptr, PairSys s0, PairSys s1) {
+    const bool on0 = s0.scal[S_DONE] == 0.0 && s0.scal[S_STOP] == 0.0;
+    const bool on1 = s1.scal[S_DONE] == 0.0 && s1.scal[S_STOP] == 0.0;
+    if (!on0 && !on1) return;
+    __shared__ double sA[kSpChunk];
+    __shared__ int sI[kSpChunk];
+    const double beta0 = s0.scal[S_BETA], beta1 = s1.scal[S_BETA];
+    const int i0 = blockIdx.x * blockDim.x, i = i0 + threadIdx.x;
+    const bool act = i < n;
+    const bool row0 = act && !s0.fixflag[i], row1 = act && !s1.fixflag[i];
+    const bool row = (on0 && row0) || (on1 && row1);
+    double dot0 = 0.0, dot1 = 0.0;
+    staged_rows(A, idx, rowptr[i0], rowptr[min(i0 + (int)blockDim.x, n)], row ? rowptr[i] : 0,
+                row ? rowptr[i + 1] : 0, sA, sI, [&](double a, int c) {
+                    if (on0 && row0) dot0 = dot0 + a * (beta0 * s0.pold[c] + s0.z[c]);
+                    if (on1 && row1) dot1 = dot1 + a * (beta1 * s1.pold[c] + s1.z[c]);
+                });
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const PairSys& q = k ? s1 : s0;
+        if (!(k ? on1 : on0)) continue;  // uniform over the grid
+        double v = 0.0;
+        if (act) {
+            const double pi = (k ? beta1 : beta0) * q.pold[i] + q.z[i];
+            const double yi = (k ? row1 : row0) ? (k ? dot1 : dot0) : 1.0e30 * pi;
+            q.pnew[i] = pi;
+            q.y[i] = yi;
+            v = pi * yi;
+        }
+        block_partial(v, OpSum(), q.partial);
+    }
+}
+
+// block k finalizes system k (k_fin_b's body)
+__global__ void __launch_bounds__(256)
+k_fin_b2(PairSys s0, PairSys s1, int nb) {
+    const PairSys& q = blockIdx.x ? s1 : s0;
+    double* scal = q.scal;
+    if (scal[S_DONE] != 0.0 || scal[S_STOP] != 0.0) return;
+    const double py = combine_partials(q.partial, nb, 0.0, OpSum());
+    if (threadIdx.x == 0) {
+        double e = scal[S_ENEW];
+        scal[S_PY] = py;
+        scal[S_ALFA] = e / py;
+        scal[S_EOLD] = e;
+        bool more = fabs(e) > scal[S_TOLZ] && scal[S_K] < scal[S_MAXIT];
+        if (!more) scal[S_STOP] = 1.0;
+    }
+}
+
 __global__ void k_store_result(const double* __restrict__ scal, double* __restrict__ iters,
                                double* __restrict__ rz, double* __restrict__ itsum) {
     *iters = scal[S_K];
@@ -369,10 +483,33 @@ void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A
         }
         return all;
     };
+    // two systems: the paired kernels (both p buffers flip together; a
+    // system that has stopped or finished is skipped on the device)
+    auto pair = [&](int k) {
+        PcgWork& w = *sys[k].w;
+        double* pbuf[2] = {w.pa.p, w.pb.p};
+        return PairSys{sys[k].x, w.r.p, w.z.p, w.y.p, w.partial.p, w.scal.p, pbuf[cur[k]],
+                       pbuf[cur[k]], pbuf[cur[k] ^ 1], w.fixflag.p};
+    };
     // most warm-started mesh solves stop at the first residual test; batches
     // grow while a solve runs on (few polls, few no-op launches)
     int chunk = 4;
     for (long launched = 0; !poll(); launched += chunk) {
+        if (nsys == 2 && !done[0] && !done[1] && cur[0] == cur[1]) {
+            for (int q = 0; q < chunk; ++q) {
+                const PairSys p0 = pair(0), p1 = pair(1);
+                k_iter_a2<<<g, blk, 0, s>>>(npoin, p0, p1, diag);
+                k_fin_a2<<<dim3(2), fin, 0, s>>>(p0, p1, nb);
+                k_iter_b2<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, p0, p1);
+                k_fin_b2<<<dim3(2), fin, 0, s>>>(p0, p1, nb);
+                cur[0] ^= 1;
+                cur[1] ^= 1;
+            }
+            CFD_HIP(hipGetLastError());
+            CFD_CHECK(launched <= 4L * maxit + 64, "PCG did not terminate");
+            chunk = std::min(chunk * 2, 16);
+            continue;
+        }
         for (int k = 0; k < nsys; ++k)
             if (!done[k] && stop[k]) iter_a(k);  // the final x update ends the solve
         for (int q = 0; q < chunk; ++q) {
