@@ -734,8 +734,13 @@ struct PatchDev {
     const Consts* cases;
 };
 
+// At most 168 VGPRs, so that 3 waves per SIMD stay resident (the LDS of the
+// default patch shapes allows 3 blocks of 256 threads per CU): the FP64
+// fixed-mesh stages fit without spilling (174 VGPRs would leave 2 waves),
+// the FP64 moving-mesh stages spill 40 B per thread and still run faster
+// than at 188 VGPRs and 2 waves
 template <class R, bool kFuente, bool kFinal>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3)))
 k_patch_stage(PatchDev pd, ElemConstsP<R> kc, PatchIn<R> in, StageDev a) {
     using V4 = typename Vec4<R>::type;
     extern __shared__ double4 smem4[];
