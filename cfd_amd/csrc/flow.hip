@@ -829,15 +829,35 @@ k_halo_stage(const int* __restrict__ hnode, const int* __restrict__ hptr,
     if (h >= nhalo) return;
     if (hcase) a.fr = cases[hcase[h]].fr;  // batches: the node's case
     const int g = hnode[h] - 1;
-    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    for (int j = hptr[h]; j < hptr[h + 1]; ++j) {
-        const double4 v = ld4(hpart + 4 * (long long)hidx[j]);
-        r0 = r0 + v.x;
-        r1 = r1 + v.y;
-        r2 = r2 + v.z;
-        r3 = r3 + v.w;
+    const double4 u = ld4(U + 4 * (long long)g);
+    const int j0 = hptr[h], j1 = hptr[h + 1];
+    // the first kHaloBatch entries' loads all in flight at once (a node lies
+    // in 2-4 patches as a rule), then the sums in entry order
+    constexpr int kHaloBatch = 4;
+    double4 v[kHaloBatch];
+#pragma unroll
+    for (int q = 0; q < kHaloBatch; ++q) {
+        const int j = j0 + q;
+        v[q] = j < j1 ? ld4(hpart + 4 * (long long)hidx[j]) : make_double4(0.0, 0.0, 0.0, 0.0);
     }
-    stage_node<kFinal>(a, g, ld4(U + 4 * (long long)g), r0, r1, r2, r3);
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+#pragma unroll
+    for (int q = 0; q < kHaloBatch; ++q) {
+        if (j0 + q < j1) {
+            r0 = r0 + v[q].x;
+            r1 = r1 + v[q].y;
+            r2 = r2 + v[q].z;
+            r3 = r3 + v[q].w;
+        }
+    }
+    for (int j = j0 + kHaloBatch; j < j1; ++j) {
+        const double4 w = ld4(hpart + 4 * (long long)hidx[j]);
+        r0 = r0 + w.x;
+        r1 = r1 + w.y;
+        r2 = r2 + w.z;
+        r3 = r3 + w.w;
+    }
+    stage_node<kFinal>(a, g, u, r0, r1, r2, r3);
 }
 
 // b[3q+k] = (R)a[3 perm[q] + k]

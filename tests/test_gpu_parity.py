@@ -605,3 +605,19 @@ def test_session_rejects_patch_shapes_beyond_lds():
     S.step(2)
     assert np.isfinite(S.get("U")).all()
     S.close()
+
+
+def test_session_rejects_bad_modes():
+    """Out-of-range precision and warm-start modes are refused at creation."""
+    from cfd_amd._lib import CfdError
+    from cfd_amd.solver import Session
+    cs = ellipse_case(8, 24)
+    for kw in ({"precision": 3}, {"precision": -1}, {"warm_start": 3}, {"warm_start": -1}):
+        with pytest.raises(CfdError):
+            Session(cs, **kw)
+    for ws in (1, 2):   # the learned start and its ablation both run
+        S = Session(cs, pcg_rel=1, pcg_tol=1e-8, warm_start=ws)
+        S.step(4)
+        sc = S.scalars()
+        assert np.isfinite(S.get("X")).all() and sc["WS_SOLVES"] == 4
+        S.close()
