@@ -751,11 +751,11 @@ struct cfd_session {
         const bool fu = fuente_needed();
         tb(CFD_K_ESTAB);
         if (d.precision == 1)
-            launch_estab_patch(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
-                               elem32(), s);
+            launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
+                               DNY.p, nullptr, nullptr, dtmin, elem32(), s);
         else if (d.precision == 0)
-            launch_estab_patch(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
-                               elem64(), s);
+            launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
+                               DNY.p, DNX64.p, DNY64.p, dtmin, elem64(), s);
         else
             launch_estab(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
                          SHOC.p, T1.p, T2.p, T3.p, s);

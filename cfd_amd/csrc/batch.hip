@@ -132,11 +132,11 @@ struct cfd_batch {
         launch_step_ctl(partials.p, pb.p, nchunks, ctl.p, ncases, itlocal, 0, 0.0, 0.0, s);
         launch_dt_finish(DT.p, DTL.p, ctl.p, ecase.p, itlocal != 0, ne, s);
         if (precision == 1)
-            launch_estab_patch(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
-                               elem32(), s);
+            launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
+                               DNY.p, nullptr, nullptr, dtmin, elem32(), s);
         else
-            launch_estab_patch(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
-                               elem64(), s);
+            launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
+                               DNY.p, DNX64.p, DNY64.p, dtmin, elem64(), s);
         PatchIn<float> in32{U(), T.p, nullptr, nullptr, DNX32.p, DNY32.p,
                             FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
         PatchIn<double> in64{U(), T.p, nullptr, nullptr, DNX64.p, DNY64.p,

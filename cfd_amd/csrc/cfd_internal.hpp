@@ -209,12 +209,17 @@ void launch_estab(const Tables& t, const Consts& c, const double* U, const doubl
                   const double* GAMM, const double* VX, const double* VY, const double* WX,
                   const double* WY, const double* dNx, const double* dNy, const double* dtmin,
                   double* shoc, double* t1, double* t2, double* t3, hipStream_t s);
-// The same arithmetic, writing the patch-order inputs o of the fused kernel
+// The same arithmetic in patch order, writing the patch-order inputs o of
+// the fused kernel: one block per patch of ps with the patch's node values
+// in LDS.  dNx_patch/dNy_patch: the patch-order FP64 gradients when they
+// exist (the fused FP64 path), else null (read dNx/dNy through ps.perm).
+struct PatchSet;
 template <class R>
-void launch_estab_patch(const Tables& t, const Consts& c, const double* U, const double* T,
-                        const double* GAMM, const double* VX, const double* VY, const double* WX,
-                        const double* WY, const double* dNx, const double* dNy,
-                        const double* dtmin, const ElemP<R>& o, hipStream_t s);
+void launch_estab_patch(const Tables& t, const PatchSet& ps, const Consts& c, const double* U,
+                        const double* T, const double* GAMM, const double* VX, const double* VY,
+                        const double* WX, const double* WY, const double* dNx, const double* dNy,
+                        const double* dNx_patch, const double* dNy_patch, const double* dtmin,
+                        const ElemP<R>& o, hipStream_t s);
 // SHOC, T_SUGN1..3 (element order, FP64) from the patch-order copies of o
 template <class R>
 void launch_estab_widen(const ElemP<R>& o, const int* inv_perm, int nelem, double* shoc,

@@ -1,7 +1,7 @@
 // Flow kernels of the explicit SUPG step.
 //
 //  k_deltat / k_deltat_clip  subrutinas.f90:155-218   element dt, global min, 10x clip
-//  k_estab                   subrutinas.f90:332-446   tau_SUGN1..3 and shock delta
+//  k_estab, k_estab_patch    subrutinas.f90:332-446   tau_SUGN1..3 and shock delta
 //  k_element_rhs             calcRHS.f90:36-151 (+ FUENTE subrutinas.f90:1060-1091)
 //  k_gather_rhs              the atomic scatters of calcRHS/FUENTE as a gather
 //  k_stage                   gather + RK stage update + primitives + BCs
@@ -82,46 +82,32 @@ __global__ void k_deltat_clip(double* __restrict__ DT, const double* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// ESTAB.  kPatch = false: the FP64 outputs SHOC, T_SUGN1..3 per element.
-// kPatch = true (fused paths): writes only the inputs of the fused residual,
-// rounded to R, at the element's patch position (the reads stay in element
-// order and coalesced; the writes scatter).
+// ESTAB: k_estab in element order (the FP64 outputs SHOC, T_SUGN1..3 per
+// element); k_estab_patch (below, with the patch structures) in patch order
+// for the fused paths.  Both run estab_elem, so both give the same values.
 // ---------------------------------------------------------------------------
-template <class R, bool kPatch>
-__global__ void __launch_bounds__(kBlock)
-k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const double* __restrict__ T,
-        const double* __restrict__ GAMM, const double* __restrict__ VEL_X,
-        const double* __restrict__ VEL_Y, const double* __restrict__ W_X,
-        const double* __restrict__ W_Y, const double* __restrict__ dNx,
-        const double* __restrict__ dNy, int nelem, double fr_, double tinf_, double rhoinf_,
-        const double* __restrict__ dtminp, double* __restrict__ shoc, double* __restrict__ t1,
-        double* __restrict__ t2, double* __restrict__ t3, ElemP<R> o) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nelem) return;
-    const int q = kPatch ? o.inv_perm[e] : e;
-    const int cs = (kPatch && o.ecase) ? o.ecase[e] : 0;
-    const double dtmin = dtminp[(size_t)cs * o.dt_stride];
-    const Consts* cq = (kPatch && o.cases) ? o.cases + cs : nullptr;
-    const double fr = cq ? cq->fr : fr_, tinf = cq ? cq->t_inf : tinf_;
-    const double rinf = cq ? cq->rho_inf : rhoinf_, cte = cq ? cq->cte : o.cte;
-    int n1 = inpoel[3 * e] - 1, n2 = inpoel[3 * e + 1] - 1, n3 = inpoel[3 * e + 2] - 1;
-    double DNX[3] = {dNx[3 * e], dNx[3 * e + 1], dNx[3 * e + 2]};
-    double DNY[3] = {dNy[3 * e], dNy[3 * e + 1], dNy[3 * e + 2]};
-    double gm = (GAMM[n1] + GAMM[n2] + GAMM[n3]) / 3.0;
+// The element's SUPG parameters from its three nodes' values (node k of the
+// element in its connectivity order): shock capturing sh and tau1..3.
+__device__ inline void estab_elem(const double DNX[3], const double DNY[3], const double rho[3],
+                                  const double Tn[3], const double vxn[3], const double vyn[3],
+                                  const double wxn[3], const double wyn[3], const double gmn[3],
+                                  double fr, double tinf, double rinf, double dtmin, double& sh,
+                                  double& rrr, double& s2, double& s3) {
+    double gm = (gmn[0] + gmn[1] + gmn[2]) / 3.0;
     double tau = 0.0, h_rgne = 0.0, h_rgn = 0.0, h_jgn = 0.0;
-    double r1 = U[4 * n1], r2 = U[4 * n2], r3 = U[4 * n3];
+    double r1 = rho[0], r2 = rho[1], r3 = rho[2];
     double rho_elem = (r1 + r2 + r3) / 3.0;
-    double vx = (VEL_X[n1] + VEL_X[n2] + VEL_X[n3]) / 3.0;
-    double vy = (VEL_Y[n1] + VEL_Y[n2] + VEL_Y[n3]) / 3.0;
-    double wx = (W_X[n1] + W_X[n2] + W_X[n3]) / 3.0;
-    double wy = (W_Y[n1] + W_Y[n2] + W_Y[n3]) / 3.0;
+    double vx = (vxn[0] + vxn[1] + vxn[2]) / 3.0;
+    double vy = (vyn[0] + vyn[1] + vyn[2]) / 3.0;
+    double wx = (wxn[0] + wxn[1] + wxn[2]) / 3.0;
+    double wy = (wyn[0] + wyn[1] + wyn[2]) / 3.0;
     vx = vx - wx;
     vy = vy - wy;
     double vel2 = sqrt(vx * vx + vy * vy);
     double drx = r1 * DNX[0] + r2 * DNX[1] + r3 * DNX[2];
     double dry = r1 * DNY[0] + r2 * DNY[1] + r3 * DNY[2];
     double dr2 = sqrt(drx * drx + dry * dry) + 1.0e-20;
-    double T1 = T[n1], T2 = T[n2], T3 = T[n3];
+    double T1 = Tn[0], T2 = Tn[1], T3 = Tn[2];
     double dtx = T1 * DNX[0] + T2 * DNX[1] + T3 * DNX[2];
     double dty = T1 * DNY[0] + T2 * DNY[1] + T3 * DNY[2];
     double dt2 = sqrt(dtx * dtx + dty * dty) + 1.0e-20;
@@ -156,28 +142,51 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
     if (h_jgn > 10.0) h_jgn = 0.0;
     double tr1 = dr2 * h_jgn / rho_elem;
     double zzz = h_jgn / (2.0 * cc);
-    const double sh = (tr1 + tr1 * tr1) * 0.5 * (cc * cc) * zzz;
+    sh = (tr1 + tr1 * tr1) * 0.5 * (cc * cc) * zzz;
     double resumen = 1.0 / (tau * tau) + (2.0 / dtmin) * (2.0 / dtmin);
-    double rrr = pow_m1_2(resumen);
-    double s2 = rrr, s3 = rrr;
+    rrr = pow_m1_2(resumen);
+    s2 = rrr;
+    s3 = rrr;
     if (fmu != 0.0) {
         double ts3 = (h_rgn * h_rgn) / (4.0 * fmu / rinf);
         double ts3e = (h_rgne * h_rgne) / (4.0 * fmu / rinf);
         s2 = pow_m1_2(resumen + 1.0 / (ts3 * ts3));
         s3 = pow_m1_2(resumen + 1.0 / (ts3e * ts3e));
     }
-    if (kPatch) {  // inputs of the fused element residual, rounded once here
-        o.nu[q] = (R)(sh * cte);
-        o.ta[q] = (R)rrr;
-        o.tb[q] = (R)s2;
-        o.tc[q] = (R)s3;
-        o.fac[q] = (R)(o.area[e] * o.dtl[e] / 3.0);
-    } else {
-        shoc[e] = sh;
-        t1[e] = rrr;
-        t2[e] = s2;
-        t3[e] = s3;
+}
+
+// Element order: the FP64 outputs SHOC, T_SUGN1..3 per element
+__global__ void __launch_bounds__(kBlock)
+k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const double* __restrict__ T,
+        const double* __restrict__ GAMM, const double* __restrict__ VEL_X,
+        const double* __restrict__ VEL_Y, const double* __restrict__ W_X,
+        const double* __restrict__ W_Y, const double* __restrict__ dNx,
+        const double* __restrict__ dNy, int nelem, double fr, double tinf, double rinf,
+        const double* __restrict__ dtminp, double* __restrict__ shoc, double* __restrict__ t1,
+        double* __restrict__ t2, double* __restrict__ t3) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    const int n[3] = {inpoel[3 * e] - 1, inpoel[3 * e + 1] - 1, inpoel[3 * e + 2] - 1};
+    const double DNX[3] = {dNx[3 * e], dNx[3 * e + 1], dNx[3 * e + 2]};
+    const double DNY[3] = {dNy[3 * e], dNy[3 * e + 1], dNy[3 * e + 2]};
+    double rho[3], Tn[3], vxn[3], vyn[3], wxn[3], wyn[3], gmn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        rho[k] = U[4 * n[k]];
+        Tn[k] = T[n[k]];
+        vxn[k] = VEL_X[n[k]];
+        vyn[k] = VEL_Y[n[k]];
+        wxn[k] = W_X[n[k]];
+        wyn[k] = W_Y[n[k]];
+        gmn[k] = GAMM[n[k]];
     }
+    double sh, rrr, s2, s3;
+    estab_elem(DNX, DNY, rho, Tn, vxn, vyn, wxn, wyn, gmn, fr, tinf, rinf, *dtminp, sh, rrr, s2,
+               s3);
+    shoc[e] = sh;
+    t1[e] = rrr;
+    t2[e] = s2;
+    t3[e] = s3;
 }
 
 // SHOC, T_SUGN1..3 from the patch-order copies (fused paths): element e's
@@ -860,6 +869,75 @@ k_halo_stage(const int* __restrict__ hnode, const int* __restrict__ hptr,
     stage_node<kFinal>(a, g, u, r0, r1, r2, r3);
 }
 
+// ESTAB in patch order (fused paths): one patch per block (or, resident
+// grid, patches blockIdx.x + k gridDim.x); the patch's node values in LDS;
+// element q's parameters, rounded to R, written at q (coalesced).  FP64
+// gradients: the patch-order copy when the element type is double (an exact
+// copy), else the element-order arrays through perm.
+struct EstabNode {
+    double rho, T, vx, vy, wx, wy, gm;
+};
+struct EstabIn {
+    const double *U, *T, *GAMM, *VX, *VY, *WX, *WY;
+    const double *dNx, *dNy;      // element order
+    const double *dNxp, *dNyp;    // patch order (null: read dNx/dNy through perm)
+    const int* perm;
+    const double* dtmin;          // + case * dt_stride
+    double fr, tinf, rinf;
+};
+
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
+    extern __shared__ double4 smem4[];
+    EstabNode* sN = reinterpret_cast<EstabNode*>(smem4);  // maxn
+    for (int p = blockIdx.x; p < pd.npatch; p += gridDim.x) {
+        const int cs = pd.pcase ? pd.pcase[p] : 0;
+        const Consts* cq = pd.cases ? pd.cases + cs : nullptr;
+        const double fr = cq ? cq->fr : ei.fr, tinf = cq ? cq->t_inf : ei.tinf;
+        const double rinf = cq ? cq->rho_inf : ei.rinf, cte = cq ? cq->cte : o.cte;
+        const double dtmin = ei.dtmin[(size_t)cs * o.dt_stride];
+        const int nb = pd.node_base[p], nl = pd.node_base[p + 1] - nb;
+        const long long e0 = pd.elem_base[p];
+        const int nep = pd.elem_base[p + 1] - pd.elem_base[p];
+        for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+            const int g = pd.pnode[nb + l] - 1;
+            sN[l] = EstabNode{ei.U[4 * (long long)g], ei.T[g], ei.VX[g], ei.VY[g],
+                              ei.WX[g], ei.WY[g], ei.GAMM[g]};
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nep; q += blockDim.x) {
+            const long long e = e0 + q;
+            const uint64_t w = (uint64_t)pd.loc[2 * e] | ((uint64_t)pd.loc[2 * e + 1] << 32);
+            const int ln[3] = {(int)(w & 1023u), (int)((w >> 10) & 1023u), (int)((w >> 20) & 1023u)};
+            const int eo = ei.perm[e];
+            double DNX[3], DNY[3], rho[3], Tn[3], vxn[3], vyn[3], wxn[3], wyn[3], gmn[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                DNX[k] = ei.dNxp ? ei.dNxp[3 * e + k] : ei.dNx[3 * (long long)eo + k];
+                DNY[k] = ei.dNyp ? ei.dNyp[3 * e + k] : ei.dNy[3 * (long long)eo + k];
+                const EstabNode& nd = sN[ln[k]];
+                rho[k] = nd.rho;
+                Tn[k] = nd.T;
+                vxn[k] = nd.vx;
+                vyn[k] = nd.vy;
+                wxn[k] = nd.wx;
+                wyn[k] = nd.wy;
+                gmn[k] = nd.gm;
+            }
+            double sh, rrr, s2, s3;
+            estab_elem(DNX, DNY, rho, Tn, vxn, vyn, wxn, wyn, gmn, fr, tinf, rinf, dtmin, sh, rrr,
+                       s2, s3);
+            o.nu[e] = (R)(sh * cte);
+            o.ta[e] = (R)rrr;
+            o.tb[e] = (R)s2;
+            o.tc[e] = (R)s3;
+            o.fac[e] = (R)(o.area[eo] * o.dtl[eo] / 3.0);
+        }
+        __syncthreads();  // the next patch overwrites sN
+    }
+}
+
 // b[3q+k] = (R)a[3 perm[q] + k]
 template <class R>
 __global__ void k_to_patch(const double* __restrict__ a, R* __restrict__ b,
@@ -906,21 +984,35 @@ void launch_estab(const Tables& t, const Consts& c, const double* U, const doubl
                   const double* GAMM, const double* VX, const double* VY, const double* WX,
                   const double* WY, const double* dNx, const double* dNy, const double* dtmin,
                   double* shoc, double* t1, double* t2, double* t3, hipStream_t s) {
-    k_estab<double, false><<<grid_for(t.nelem), kBlock, 0, s>>>(
-        t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy, t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin,
-        shoc, t1, t2, t3, Elem64());
+    k_estab<<<grid_for(t.nelem), kBlock, 0, s>>>(t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy,
+                                                 t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin, shoc,
+                                                 t1, t2, t3);
     CFD_HIP(hipGetLastError());
 }
 
 template <class R>
-void launch_estab_patch(const Tables& t, const Consts& c, const double* U, const double* T,
-                        const double* GAMM, const double* VX, const double* VY, const double* WX,
-                        const double* WY, const double* dNx, const double* dNy,
-                        const double* dtmin, const ElemP<R>& o, hipStream_t s) {
-    CFD_CHECK(o.nu && o.inv_perm, "internal: patch outputs missing");
-    k_estab<R, true><<<grid_for(t.nelem), kBlock, 0, s>>>(
-        t.inpoel.p, U, T, GAMM, VX, VY, WX, WY, dNx, dNy, t.nelem, c.fr, c.t_inf, c.rho_inf, dtmin,
-        nullptr, nullptr, nullptr, nullptr, o);
+void launch_estab_patch(const Tables& t, const PatchSet& ps, const Consts& c, const double* U,
+                        const double* T, const double* GAMM, const double* VX, const double* VY,
+                        const double* WX, const double* WY, const double* dNx, const double* dNy,
+                        const double* dNx_patch, const double* dNy_patch, const double* dtmin,
+                        const ElemP<R>& o, hipStream_t s) {
+    CFD_CHECK(o.nu && ps.perm.p, "internal: patch outputs missing");
+    CFD_CHECK(ps.max_nodes <= kPatchMaxNodes, "internal: patch beyond the node budget");
+    PatchDev pd{};
+    pd.loc = ps.loc.p;
+    pd.elem_base = ps.elem_base.p;
+    pd.node_base = ps.node_base.p;
+    pd.pnode = ps.pnode.p;
+    pd.nelem = t.nelem;
+    pd.maxn = ps.max_nodes;
+    pd.pe = ps.pe;
+    pd.npatch = ps.npatch;
+    pd.pcase = ps.cases ? ps.pcase.p : nullptr;
+    pd.cases = ps.cases;
+    EstabIn ei{U, T, GAMM, VX, VY, WX, WY, dNx, dNy, dNx_patch, dNy_patch, ps.perm.p, dtmin,
+               c.fr, c.t_inf, c.rho_inf};
+    const size_t lds = (size_t)ps.max_nodes * sizeof(EstabNode);
+    k_estab_patch<R><<<ps.npatch, kBlock, lds, s>>>(pd, ei, o);
     CFD_HIP(hipGetLastError());
 }
 
@@ -932,12 +1024,14 @@ void launch_estab_widen(const ElemP<R>& o, const int* inv_perm, int nelem, doubl
     CFD_HIP(hipGetLastError());
 }
 
-template void launch_estab_patch<float>(const Tables&, const Consts&, const double*, const double*,
+template void launch_estab_patch<float>(const Tables&, const PatchSet&, const Consts&,
                                          const double*, const double*, const double*,
                                          const double*, const double*, const double*,
-                                         const double*, const double*, const ElemP<float>&,
-                                         hipStream_t);
-template void launch_estab_patch<double>(const Tables&, const Consts&, const double*,
+                                         const double*, const double*, const double*,
+                                         const double*, const double*, const double*,
+                                         const ElemP<float>&, hipStream_t);
+template void launch_estab_patch<double>(const Tables&, const PatchSet&, const Consts&,
+                                          const double*, const double*, const double*,
                                           const double*, const double*, const double*,
                                           const double*, const double*, const double*,
                                           const double*, const double*, const double*,
