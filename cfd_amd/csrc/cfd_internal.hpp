@@ -127,6 +127,18 @@ inline unsigned grid_for(size_t n, int block = kBlock) {
     return (unsigned)(g ? g : 1);
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn (block b to XCD b mod 8),
+// and each XCD has its own L2.  xcd_block(b, n) gives block b the logical
+// block that makes XCD x work through the x-th eighth of [0, n) in order, so
+// neighbouring logical blocks (neighbouring patches, neighbouring row
+// blocks) mostly share an L2.  A bijection of [0, n); work and partial
+// sums indexed by the logical block keep every result unchanged.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int q = n / kXcds, r = n % kXcds, x = b % kXcds, j = b / kXcds;
+    return x * q + (x < r ? x : r) + j;
+}
+
 // ---------------------------------------------------------------------------
 // Mesh tables (built on the device by tables.hip)
 // ---------------------------------------------------------------------------

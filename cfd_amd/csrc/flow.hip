@@ -743,17 +743,6 @@ struct PatchDev {
     const Consts* cases;
 };
 
-// Workgroups are dispatched to the 8 XCDs in turn (block b to XCD b mod 8),
-// and each XCD has its own L2.  With one block per patch, block b takes
-// patch xcd_patch(b): XCD x works through the x-th eighth of the patches in
-// order, so neighbouring patches (Morton order), which share halo nodes,
-// mostly meet in the same L2.  A bijection of [0, n).
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_patch(int b, int n) {
-    const int q = n / kXcds, r = n % kXcds, x = b % kXcds, j = b / kXcds;
-    return x * q + min(x, r) + j;
-}
-
 // At most 168 VGPRs, so that 3 waves per SIMD stay resident (the LDS of the
 // default patch shapes allows 3 blocks of 256 threads per CU): the FP64
 // fixed-mesh stages fit without spilling (174 VGPRs would leave 2 waves),
@@ -775,7 +764,7 @@ k_patch_stage(PatchDev pd, ElemConstsP<R> kc, PatchIn<R> in, StageDev a) {
         // measured: the XCD order helps the FP32 stages (149 -> 143 us per
         // launch at 4.2M triangles) and slows the FP64 ones (204 -> 208 us)
         const int p = sizeof(R) == sizeof(float) && (int)gridDim.x == pd.npatch
-                          ? xcd_patch(p0, pd.npatch)
+                          ? xcd_block(p0, pd.npatch)
                           : p0;
         ElemConstsP<R> kl = kc;
         StageDev al = a;
@@ -908,7 +897,7 @@ k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
     extern __shared__ double4 smem4[];
     EstabNode* sN = reinterpret_cast<EstabNode*>(smem4);  // maxn
     for (int p0 = blockIdx.x; p0 < pd.npatch; p0 += gridDim.x) {
-        const int p = (int)gridDim.x == pd.npatch ? xcd_patch(p0, pd.npatch) : p0;
+        const int p = (int)gridDim.x == pd.npatch ? xcd_block(p0, pd.npatch) : p0;
         const int cs = pd.pcase ? pd.pcase[p] : 0;
         const Consts* cq = pd.cases ? pd.cases + cs : nullptr;
         const double fr = cq ? cq->fr : ei.fr, tinf = cq ? cq->t_inf : ei.tinf;

@@ -25,10 +25,12 @@
 //
 // Two systems on one matrix (the x and y mesh solves) iterate in lockstep
 // through paired kernels: one launch per kernel for both, with the matrix
-// entries staged once for both products.  Each system's arithmetic, block
-// decomposition and partial order are those of the single-system kernels,
-// so the paired iterations are bit for bit the single ones; a system that
-// has finished is skipped inside the paired kernels.
+// entries staged once for both products, and row blocks in XCD order
+// (xcd_block: a row's neighbours' blocks mostly on the same XCD).  Each
+// system's arithmetic, logical block decomposition and partial order are
+// those of the single-system kernels, so the paired iterations are bit for
+// bit the single ones; a system that has finished is skipped inside the
+// paired kernels.
 //
 // Relative mode (rel = 1): the two tests use tol^2 times the r.r and r.z of
 // the residual of the guess that is zero off the fixed nodes, so the stopping
@@ -283,7 +285,8 @@ struct PairSys {
 
 __global__ void __launch_bounds__(kBlock)
 k_iter_a2(int n, PairSys s0, PairSys s1, const double* __restrict__ diag) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);  // logical row block
+    const int i = lb * blockDim.x + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const PairSys& q = k ? s1 : s0;
@@ -302,7 +305,7 @@ k_iter_a2(int n, PairSys s0, PairSys s1, const double* __restrict__ diag) {
             q.z[i] = zi;
             v = ri * zi;
         }
-        block_partial(v, OpSum(), q.partial);
+        block_partial(v, OpSum(), q.partial, lb);
     }
 }
 
@@ -337,7 +340,8 @@ k_iter_b2(int n, const double* __restrict__ A, const int* __restrict__ idx,
     __shared__ double sA[kSpChunk];
     __shared__ int sI[kSpChunk];
     const double beta0 = s0.scal[S_BETA], beta1 = s1.scal[S_BETA];
-    const int i0 = blockIdx.x * blockDim.x, i = i0 + threadIdx.x;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);  // logical row block
+    const int i0 = lb * blockDim.x, i = i0 + threadIdx.x;
     const bool act = i < n;
     const bool row0 = act && !s0.fixflag[i], row1 = act && !s1.fixflag[i];
     const bool row = (on0 && row0) || (on1 && row1);
@@ -359,7 +363,7 @@ k_iter_b2(int n, const double* __restrict__ A, const int* __restrict__ idx,
             q.y[i] = yi;
             v = pi * yi;
         }
-        block_partial(v, OpSum(), q.partial);
+        block_partial(v, OpSum(), q.partial, lb);
     }
 }
 

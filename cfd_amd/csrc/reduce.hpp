@@ -42,11 +42,16 @@ __device__ inline double block_reduce(double v, Op op) {
     return v;
 }
 
-// Full-grid pass: one partial per block.
+// Full-grid pass: one partial per block (at index `at`: the block's logical
+// index when the grid is remapped, else blockIdx.x).
+template <class Op>
+__device__ inline void block_partial(double v, Op op, double* partials, int at) {
+    v = block_reduce(v, op);
+    if (threadIdx.x == 0) partials[at] = v;
+}
 template <class Op>
 __device__ inline void block_partial(double v, Op op, double* partials) {
-    v = block_reduce(v, op);
-    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    block_partial(v, op, partials, (int)blockIdx.x);
 }
 
 // One-block finalize: combine partials[0..n); valid in thread 0.
