@@ -39,8 +39,11 @@ k_deltat(const int* __restrict__ inpoel, const double* __restrict__ area, const 
         t_inf = q.t_inf;
         fsafe = q.fsafe;
     }
+    // grid-stride: logical blocks in XCD order (neighbouring elements' nodes
+    // mostly in one L2); the min partial goes to the logical block
+    const int lb = chunk ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const int beg = chunk ? chunk[blockIdx.x] + (int)threadIdx.x
-                          : (int)(blockIdx.x * blockDim.x + threadIdx.x);
+                          : (int)(lb * blockDim.x + threadIdx.x);
     const int end = chunk ? chunk[blockIdx.x + 1] : nelem;
     const int step = chunk ? (int)blockDim.x : (int)(gridDim.x * blockDim.x);
     double dmin = 1.0e20;
@@ -71,7 +74,7 @@ k_deltat(const int* __restrict__ inpoel, const double* __restrict__ area, const 
         DT[e] = dtelem;
         if (dtelem < dmin) dmin = dtelem;
     }
-    block_partial(dmin, OpMin(), partials);
+    block_partial(dmin, OpMin(), partials, lb);
 }
 
 __global__ void k_deltat_clip(double* __restrict__ DT, const double* __restrict__ dtmin, int nelem) {
@@ -839,7 +842,7 @@ k_halo_stage(const int* __restrict__ hnode, const int* __restrict__ hptr,
              const int* __restrict__ hidx, const double* __restrict__ hpart,
              const double* __restrict__ U, StageDev a, int nhalo, const int* __restrict__ hcase,
              const Consts* __restrict__ cases) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    const int h = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (h >= nhalo) return;
     if (hcase) a.fr = cases[hcase[h]].fr;  // batches: the node's case
     const int g = hnode[h] - 1;
