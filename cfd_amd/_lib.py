@@ -55,7 +55,7 @@ class CaseDesc(C.Structure):
                 ("motion_amp", C.c_double), ("motion_omega", C.c_double), ("pcg_rel", C.c_int),
                 ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int), ("gcl", C.c_int),
                 ("precision", C.c_int), ("warm_start", C.c_int), ("patch_elems", C.c_int),
-                ("patch_block", C.c_int)]
+                ("patch_block", C.c_int), ("renumber", C.c_int)]
 
 
 def _enum(name_prefix: str) -> dict:

@@ -543,7 +543,22 @@ int cfd_gcl_main(cfd_mesh* m, double* M, const double* W_x, const double* W_y, c
 // FUENTE adds exactly +-0 while the mesh velocity is identically zero
 // (fixed mesh, no moving nodes, W never set by the caller); it is skipped then.
 // ---------------------------------------------------------------------------
+// Patch-order numbering inside a session (cfd_case_desc.renumber = 1): the
+// session runs on its own copy of the mesh renumbered so that each patch's
+// elements and interior nodes are contiguous; fields cross the boundary in
+// the caller's numbering.
+struct Renumbering {
+    cfd_mesh* mesh = nullptr;       // the renumbered mesh, owned
+    std::vector<int> node_old;      // node_old[i]: the caller's node at position i
+    std::vector<int> elem_old;      // elem_old[q]: the caller's element at position q
+    std::vector<long long> lap_old; // the caller's CSR entry of each Laplace entry
+    ~Renumbering() {
+        if (mesh) cfd_mesh_destroy(mesh);
+    }
+};
+
 struct cfd_session {
+    std::unique_ptr<Renumbering> rn;  // first: destroyed after everything using its mesh
     cfd_mesh* m = nullptr;
     hipStream_t s = nullptr;
     Consts c{};
@@ -874,6 +889,111 @@ struct cfd_session {
 
 extern "C" {
 
+}  // extern "C"
+
+// A session on the patch-order renumbering of m (cfd_case_desc.renumber = 1):
+// the renumbered mesh and description, the session on them, and the maps
+// back to the caller's numbering.
+static cfd_session* create_renumbered(cfd_mesh* m, const cfd_case_desc* d) {
+    const int np = m->t.npoin, ne = m->t.nelem;
+    hipStream_t s = m->s;
+    std::vector<int> inp(3 * (size_t)ne);
+    m->t.inpoel.download(inp.data(), inp.size(), s);
+    CFD_HIP(hipStreamSynchronize(s));
+    const int pe = d->patch_elems > 0 ? d->patch_elems
+                                      : (d->precision == 1 ? kPatchElems : kPatchElems64);
+    CFD_ARG(pe >= 1 && pe <= 682, "patch_elems must be 1..682");
+    auto rn = std::make_unique<Renumbering>();
+    patch_numbering(inp.data(), d->X, d->Y, ne, np, pe, rn->elem_old, rn->node_old);
+    CFD_CHECK((int)rn->node_old.size() == np && (int)rn->elem_old.size() == ne,
+              "internal: patch numbering");
+    std::vector<int> node_new(np), elem_new(ne);
+    for (int i = 0; i < np; ++i) node_new[rn->node_old[i]] = i;
+    for (int q = 0; q < ne; ++q) elem_new[rn->elem_old[q]] = q;
+    std::vector<int> inp2(3 * (size_t)ne);
+    for (int q = 0; q < ne; ++q)
+        for (int k = 0; k < 3; ++k)  // each element keeps its node order
+            inp2[3 * (size_t)q + k] = node_new[inp[3 * (size_t)rn->elem_old[q] + k] - 1] + 1;
+    if (cfd_mesh_create(&rn->mesh, ne, np, inp2.data()) != CFD_OK)
+        throw Error(std::string("mesh: renumbered mesh: ") + g_last_error);
+    // per-node sums (MASAS, laplace, GCL) in the caller's element order, as
+    // the caller-numbered session sums them
+    reorder_esup(rn->mesh->t, rn->elem_old, rn->mesh->s);
+    // the description in the new numbering (node lists mapped, order kept)
+    auto nodes = [&](const int* a, size_t n, const char* what) {
+        std::vector<int> v(n);
+        for (size_t k = 0; k < n; ++k) {
+            CFD_ARG(a[k] >= 1 && a[k] <= np, std::string(what) + " node out of range");
+            v[k] = node_new[a[k] - 1] + 1;
+        }
+        return v;
+    };
+    cfd_case_desc d2 = *d;
+    d2.renumber = 0;
+    std::vector<double> X2(np), Y2(np);
+    for (int i = 0; i < np; ++i) {
+        X2[i] = d->X[rn->node_old[i]];
+        Y2[i] = d->Y[rn->node_old[i]];
+    }
+    d2.X = X2.data();
+    d2.Y = Y2.data();
+    const int nn = d->nmove + d->nfix_move;
+    const std::vector<int> ilaux2 = nodes(d->ilaux, nn > 0 ? nn : 0, "ilaux");
+    d2.ilaux = nn > 0 ? ilaux2.data() : nullptr;
+    std::vector<int> iset2(3 * (size_t)std::max(0, d->nsetedges));
+    for (int q = 0; q < d->nsetedges; ++q) {
+        for (int j = 0; j < 2; ++j) {
+            const int a = d->iset[3 * q + j];
+            CFD_ARG(a >= 1 && a <= np, "iset node out of range");
+            iset2[3 * q + j] = node_new[a - 1] + 1;
+        }
+        const int e = d->iset[3 * q + 2];
+        iset2[3 * q + 2] = e >= 1 && e <= ne ? elem_new[e - 1] + 1 : e;
+    }
+    d2.iset = d->nsetedges > 0 ? iset2.data() : nullptr;
+    const cfd_bc_lists& b = d->bc;
+    const std::vector<int> fv = nodes(b.ifixv_node, std::max(0, b.nfixv), "ifixv_node");
+    const std::vector<int> fr = nodes(b.ifixrho_node, std::max(0, b.nfixrho), "ifixrho_node");
+    const std::vector<int> ft = nodes(b.ifixt_node, std::max(0, b.nfixt), "ifixt_node");
+    const std::vector<int> wl = nodes(b.wall, 2 * (size_t)std::max(0, b.nwall), "wall");
+    d2.bc.ifixv_node = b.nfixv > 0 ? fv.data() : b.ifixv_node;
+    d2.bc.ifixrho_node = b.nfixrho > 0 ? fr.data() : b.ifixrho_node;
+    d2.bc.ifixt_node = b.nfixt > 0 ? ft.data() : b.ifixt_node;
+    d2.bc.wall = b.nwall > 0 ? wl.data() : b.wall;
+    cfd_session* S = nullptr;
+    if (cfd_session_create(&S, rn->mesh, &d2) != CFD_OK)
+        throw Error(std::string("arg: renumbered session: ") + g_last_error);
+    std::unique_ptr<cfd_session> own(S);
+    // each Laplace entry's position in the caller's CSR
+    const Tables& t0 = m->t;
+    const Tables& t1 = rn->mesh->t;
+    std::vector<int> rp0(np + 1), ix0(t0.nnz), rp1(np + 1), ix1(t1.nnz);
+    t0.rowptr.download(rp0.data(), np + 1, s);
+    t0.idx.download(ix0.data(), t0.nnz, s);
+    t1.rowptr.download(rp1.data(), np + 1, rn->mesh->s);
+    t1.idx.download(ix1.data(), t1.nnz, rn->mesh->s);
+    CFD_HIP(hipStreamSynchronize(s));
+    CFD_HIP(hipStreamSynchronize(rn->mesh->s));
+    CFD_CHECK(t0.nnz == t1.nnz, "internal: renumbered Laplace structure");
+    rn->lap_old.assign(t1.nnz, -1);
+    for (int i = 0; i < np; ++i) {
+        const int io = rn->node_old[i];
+        for (int q = rp1[i]; q < rp1[i + 1]; ++q) {
+            const int jo = rn->node_old[ix1[q] - 1] + 1;
+            for (int r = rp0[io]; r < rp0[io + 1]; ++r)
+                if (ix0[r] == jo) {
+                    rn->lap_old[q] = r;
+                    break;
+                }
+            CFD_CHECK(rn->lap_old[q] >= 0, "internal: renumbered Laplace entry");
+        }
+    }
+    own->rn = std::move(rn);
+    return own.release();
+}
+
+extern "C" {
+
 int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
     CFD_TRY({
         CFD_ARG(out && m && d && d->X && d->Y, "null argument");
@@ -883,6 +1003,13 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         CFD_ARG(d->precision >= 0 && d->precision <= 2,
                 "precision must be 0 (FP64), 1 (FP32 element) or 2 (FP64, reference order)");
         CFD_ARG(d->warm_start >= 0 && d->warm_start <= 2, "warm_start must be 0, 1 or 2");
+        CFD_ARG(d->renumber == 0 || d->renumber == 1, "renumber must be 0 or 1");
+        CFD_ARG(d->renumber == 0 || d->precision == 0 || d->precision == 1,
+                "renumber = 1 needs a fused path (precision 0 or 1)");
+        if (d->renumber == 1) {
+            *out = create_renumbered(m, d);
+            return CFD_OK;
+        }
         auto S = std::make_unique<cfd_session>();
         S->m = m;
         S->s = m->s;
@@ -1020,9 +1147,53 @@ int cfd_session_field_size(cfd_session* S, int f, long long* count) {
     CFD_TRY({ S->field(f, *count); })
 }
 
-int cfd_session_get_field(cfd_session* S, int f, double* host) {
+}  // extern "C"
+
+// Layout of a field for the caller's numbering: per node or per element,
+// values per entity; the Laplace values through their entry map.
+enum FieldKind { FK_NODE, FK_ELEM, FK_LAP };
+static FieldKind field_kind(int f, int& width) {
+    width = 1;
+    switch (f) {
+        case CFD_F_U: case CFD_F_U1: case CFD_F_RHS: width = 4; return FK_NODE;
+        case CFD_F_DTL: case CFD_F_DT: case CFD_F_SHOC: case CFD_F_T_SUGN1: case CFD_F_T_SUGN2:
+        case CFD_F_T_SUGN3: case CFD_F_AREA: case CFD_F_HH: case CFD_F_HHX: case CFD_F_HHY:
+            return FK_ELEM;
+        case CFD_F_DNX: case CFD_F_DNY: width = 3; return FK_ELEM;
+        case CFD_F_LAP_SPARSE: return FK_LAP;
+        default: return FK_NODE;
+    }
+}
+
+// in[position] -> out[caller's position] (to_caller) or back
+static void renumber_field(const Renumbering& rn, int f, const double* in, double* out,
+                           bool to_caller) {
+    int w = 1;
+    const FieldKind k = field_kind(f, w);
+    if (k == FK_LAP) {
+        for (size_t q = 0; q < rn.lap_old.size(); ++q) {
+            if (to_caller) out[rn.lap_old[q]] = in[q];
+            else out[q] = in[rn.lap_old[q]];
+        }
+        return;
+    }
+    const std::vector<int>& old = k == FK_NODE ? rn.node_old : rn.elem_old;
+    for (size_t i = 0; i < old.size(); ++i)
+        for (int c = 0; c < w; ++c) {
+            if (to_caller) out[w * (size_t)old[i] + c] = in[w * i + c];
+            else out[w * i + c] = in[w * (size_t)old[i] + c];
+        }
+}
+
+extern "C" {
+
+int cfd_session_get_field(cfd_session* S, int f, double* host_out) {
     CFD_TRY({
+        CFD_ARG(S && host_out, "null argument");
         long long n = 0;
+        S->field(f, n);
+        std::vector<double> tmp(S->rn ? (size_t)n : 0);
+        double* host = S->rn ? tmp.data() : host_out;
         DBuf<double>* b = S->field(f, n);
         const bool estab_field = f == CFD_F_SHOC || f == CFD_F_T_SUGN1 || f == CFD_F_T_SUGN2 ||
                                  f == CFD_F_T_SUGN3;
@@ -1034,13 +1205,18 @@ int cfd_session_get_field(cfd_session* S, int f, double* host) {
                                S->T2.p, S->T3.p, S->s);
         b->download(host, n, S->s);
         CFD_HIP(hipStreamSynchronize(S->s));
+        if (S->rn) renumber_field(*S->rn, f, tmp.data(), host_out, true);
     })
 }
 
-int cfd_session_set_field(cfd_session* S, int f, const double* host) {
+int cfd_session_set_field(cfd_session* S, int f, const double* host_in) {
     CFD_TRY({
+        CFD_ARG(S && host_in, "null argument");
         long long n = 0;
         DBuf<double>* b = S->field(f, n);
+        std::vector<double> tmp(S->rn ? (size_t)n : 0);
+        if (S->rn) renumber_field(*S->rn, f, host_in, tmp.data(), false);
+        const double* host = S->rn ? tmp.data() : host_in;
         b->upload(host, n, S->s);
         CFD_HIP(hipStreamSynchronize(S->s));
         if (f == CFD_F_W_X || f == CFD_F_W_Y || f == CFD_F_XPOS || f == CFD_F_YPOS) S->w_dirty = true;

@@ -153,6 +153,10 @@ struct Tables {
 };
 
 void build_tables(Tables& t, const int* inpoel_host, int nelem, int npoin, hipStream_t s);
+// Reorders each node's element list (esup1) by key[element] ascending and
+// rebuilds the maps that follow that order (epos, lslot): a renumbered mesh
+// then sums per node (MASAS, laplace, GCL) in the original element order.
+void reorder_esup(Tables& t, const std::vector<int>& key, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Kernel launchers (geometry, flow, PCG). All take device pointers and run on
@@ -308,6 +312,13 @@ struct PatchSet {
 };
 void build_patches(PatchSet& ps, const int* inpoel_host, const double* X_host,
                    const double* Y_host, int nelem, int npoin, int pe, int block, hipStream_t s);
+// The patch-order numbering of a mesh (host): elem_old[q] = the element
+// (0-based) at patch position q; node_old[i] = the node at new position i,
+// each patch's nodes in turn (interior nodes, then the halo nodes no earlier
+// patch holds).  A mesh renumbered by them gives patches with the same
+// elements in the same order, whose interior nodes are contiguous.
+void patch_numbering(const int* inpoel, const double* X, const double* Y, int ne, int np, int pe,
+                     std::vector<int>& elem_old, std::vector<int>& node_old);
 // A batch of disjoint meshes numbered one after the other: each case's
 // patches as build_patches makes them, concatenated
 void build_patches_batch(PatchSet& ps, int ncases, const int* const* inpoel_host,

@@ -173,6 +173,25 @@ static PatchHost patches_host(const int* inpoel, const double* X, const double* 
     return h;
 }
 
+void patch_numbering(const int* inpoel, const double* X, const double* Y, int ne, int np, int pe,
+                     std::vector<int>& elem_old, std::vector<int>& node_old) {
+    const PatchHost h = patches_host(inpoel, X, Y, ne, np, pe);
+    elem_old = h.perm;
+    node_old.clear();
+    node_old.reserve(np);
+    std::vector<char> seen(np, 0);
+    for (int p = 0; p < h.npatch; ++p)  // each patch's nodes as listed: interior first
+        for (int l = h.node_base[p]; l < h.node_base[p + 1]; ++l) {
+            const int n = h.pnode[l] - 1;
+            if (!seen[n]) {
+                seen[n] = 1;
+                node_old.push_back(n);
+            }
+        }
+    for (int n = 0; n < np; ++n)  // nodes in no element keep their relative order at the end
+        if (!seen[n]) node_old.push_back(n);
+}
+
 static void upload_patches(PatchSet& ps, const PatchHost& h, int ne, int pe, int block,
                            hipStream_t s) {
     const int npatch = h.npatch, nentries = h.nentries;

@@ -212,4 +212,31 @@ void build_tables(Tables& t, const int* inpoel_host, int nelem, int npoin, hipSt
     if (read_flag(bad, s) & 8) throw Error("mesh: Laplace slot map incomplete");
 }
 
+void reorder_esup(Tables& t, const std::vector<int>& key, hipStream_t s) {
+    const int np = t.npoin, ne = t.nelem;
+    CFD_CHECK((int)key.size() == ne, "internal: esup key size");
+    std::vector<int> e1(3 * (size_t)ne), e2(np + 1), inp(3 * (size_t)ne), epos(3 * (size_t)ne);
+    t.esup1.download(e1.data(), e1.size(), s);
+    t.esup2.download(e2.data(), e2.size(), s);
+    t.inpoel.download(inp.data(), inp.size(), s);
+    CFD_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < np; ++i) {
+        std::sort(e1.begin() + e2[i], e1.begin() + e2[i + 1],
+                  [&](int a, int b) { return key[a - 1] < key[b - 1]; });
+        for (int q = e2[i]; q < e2[i + 1]; ++q) {
+            const int e = e1[q] - 1;
+            for (int k = 0; k < 3; ++k)
+                if (inp[3 * (size_t)e + k] == i + 1) epos[3 * (size_t)e + k] = q;
+        }
+    }
+    t.esup1.upload(e1.data(), e1.size(), s);
+    t.epos.upload(epos.data(), epos.size(), s);
+    DBuf<int> bad(1);
+    bad.zero(s);
+    k_lap_slots<<<grid_for(np), kBlock, 0, s>>>(t.inpoel.p, t.esup1.p, t.esup2.p, t.rowptr.p,
+                                                t.idx.p, t.lslot.p, np, bad.p);
+    CFD_HIP(hipGetLastError());
+    if (read_flag(bad, s) & 8) throw Error("mesh: Laplace slot map incomplete");
+}
+
 }  // namespace cfd

@@ -43,6 +43,7 @@ module cfd_abi
         integer(c_int) :: pcg_rel
         real(c_double) :: pcg_tol
         integer(c_int) :: pcg_maxit, gcl, precision, warm_start, patch_elems, patch_block
+        integer(c_int) :: renumber
     end type cfd_case_desc
 
     ! one case of a batch (cfd_batch_case)

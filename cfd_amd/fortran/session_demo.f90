@@ -86,6 +86,7 @@ program session_demo
     d%warm_start = 0
     d%patch_elems = 0
     d%patch_block = 0
+    d%renumber = 0
 
     call cfd_check(cfd_mesh_create(m, nelem, npoin, inpoel), 'cfd_mesh_create')
     call cfd_check(cfd_session_create(s, m, d), 'cfd_session_create')

@@ -32,7 +32,7 @@ class Session:
                  motion_amp: float = 0.0, motion_omega: float = 0.0, pcg_tol: float = 0.0,
                  pcg_maxit: int = 0, gcl: int = -1, precision: int = 0, init: bool = True,
                  pcg_rel: int = 0, warm_start: int = 0, patch_elems: int = 0,
-                 patch_block: int = 0):
+                 patch_block: int = 0, renumber: int = 0):
         self.cs = cs
         m = cs.mesh
         self.mesh = mesh if mesh is not None else MeshContext(m.inpoel, m.npoin)
@@ -60,6 +60,7 @@ class Session:
         d.pcg_rel = pcg_rel
         d.warm_start = warm_start
         d.patch_elems, d.patch_block = patch_elems, patch_block
+        d.renumber = renumber
         self._h = C.c_void_p()
         check(lib().cfd_session_create(C.byref(self._h), self.mesh.handle, C.byref(d)))
         self.npoin, self.nelem, self.nnz = m.npoin, m.nelem, self.mesh.nnz

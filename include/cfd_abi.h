@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 3
+#define CFD_ABI_VERSION 4
 
 enum {
     CFD_OK = 0,
@@ -218,6 +218,16 @@ typedef struct {
                                 LDS (0: 512 FP32, 384 FP64)                   */
     int patch_block;         /* threads per patch, a multiple of 64 up to 512
                                 (0: 256)                                     */
+    int renumber;            /* fused paths (precision 0, 1):
+                                0: the session keeps the caller's numbering
+                                1: the session numbers nodes and elements in
+                                   patch order inside (each patch's elements
+                                   and interior nodes contiguous); fields,
+                                   boundary lists and the Laplace values go in
+                                   and out in the caller's numbering.  Fixed
+                                   meshes step bit for bit as with 0; moving
+                                   meshes differ at round-off (the PCG dot
+                                   products sum in another row order)        */
 } cfd_case_desc;
 
 enum {

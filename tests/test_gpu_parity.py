@@ -621,3 +621,74 @@ def test_session_rejects_bad_modes():
         sc = S.scalars()
         assert np.isfinite(S.get("X")).all() and sc["WS_SOLVES"] == 4
         S.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_renumbered_session_fixed_mesh_bitwise(cyl, precision):
+    """A session that numbers nodes and elements in patch order inside
+    (renumber = 1) steps a fixed mesh bit for bit as the session in the
+    caller's numbering, with every field read and written in the caller's
+    numbering (node, element and Laplace fields; a set-get round trip)."""
+    from cfd_amd.solver import Session
+    cs, o0 = cyl
+    st = o0.state()
+    A = Session(cs, precision=precision)
+    B = Session(cs, precision=precision, renumber=1)
+    for S in (A, B):
+        S.set_state(st)
+        S.set_scalars(DISN2=o0.disn2)
+    for name in ("U", "T", "DNX", "AREA", "LAP_SPARSE", "X"):
+        assert np.array_equal(B.get(name), np.asarray(st[name]).reshape(B.get(name).shape)), name
+    A.step(6)
+    B.step(6)
+    for name in ("U", "U1", "RHS", "T", "VEL_X", "P", "RMACH", "DTL", "DT", "SHOC", "T_SUGN2"):
+        assert np.array_equal(A.get(name), B.get(name)), name
+    sa, sb = A.scalars(), B.scalars()
+    for key in ("DTMIN", "TIME", "FX1", "FY1"):
+        assert sa[key] == sb[key], key
+    A.close()
+    B.close()
+    # from the free stream: the masses (MASAS) sum in the caller's element order
+    A = Session(cs, precision=precision)
+    B = Session(cs, precision=precision, renumber=1)
+    assert np.array_equal(A.get("M"), B.get("M"))
+    A.step(8)
+    B.step(8)
+    assert np.array_equal(A.get("U"), B.get("U"))
+    A.close()
+    B.close()
+
+
+def test_renumbered_session_moving_mesh_teacher_forced(ell):
+    """The pitching ellipse with the patch-order numbering inside, each step
+    from the oracle's state: within 1e-12 as the caller-numbered session (its
+    mesh solves sum their dot products in another row order).  Free runs of
+    the two numberings part at round-off and then, through ESTAB's
+    rounding-residue direction on the moving mesh, grow apart like any two
+    differently rounded runs (1e-4 after 5 steps here)."""
+    from cfd_amd.solver import Session
+    cs, o0 = ell
+    o = OracleSolver(cs)
+    o.set_state(o0.state())
+    o.disn2 = o0.disn2
+    S = Session(cs, precision=0, renumber=1)
+    for k in range(3):
+        S.set_state(o.state())
+        S.set_scalars(DISN2=o.disn2)
+        o.step()
+        S.step(1)
+        ref, got = o.state(), S.state()
+        assert got["DTMIN"] == ref["DTMIN"]
+        assert np.all(relnorm_cols(got["U"], ref["U"]) < TOL), (k, relnorm_cols(got["U"], ref["U"]))
+        for f in ("T", "VEL_X", "VEL_Y", "P", "X", "Y", "M", "LAP_SPARSE"):
+            assert relnorm(got[f], ref[f]) < TOL, (k, f)
+        assert relnorm(got["XPOS"], ref["XPOS"]) < 1e-6 or np.abs(ref["XPOS"]).max() == 0
+    assert S.scalars()["PCG_ITSUM"] > 0
+    S.close()
+
+
+def test_renumber_needs_a_fused_path():
+    from cfd_amd._lib import CfdError
+    from cfd_amd.solver import Session
+    with pytest.raises(CfdError):
+        Session(cylinder_case(12, 24), precision=2, renumber=1)
