@@ -251,12 +251,14 @@ def read_traffic(name: str):
     return None
 
 
-def run_path(cs, prec: int, steps: int, warmup: int, breakdown: bool = False) -> dict:
+def run_path(cs, prec: int, steps: int, warmup: int, breakdown: bool = False,
+             renumber: bool = True) -> dict:
     """Times `steps` whole steps after `warmup` untimed ones; HIP events on the
     session's stream time the element kernel launches inside the timed
-    region."""
+    region.  The fused paths run with the session's patch-order numbering
+    (renumber = 1: the same results bit for bit on a fixed mesh)."""
     from cfd_amd.solver import Session
-    S = Session(cs, precision=prec)
+    S = Session(cs, precision=prec, renumber=1 if renumber and prec in (0, 1) else 0)
     S.step(warmup)
     S.sync()
     S.timing(True, slots=["ELEMENT"])
@@ -304,6 +306,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--large-ale", action="store_true",
                     help="also run the 4096 x 4096 pitching ellipse (minutes)")
+    ap.add_argument("--no-renumber", action="store_true",
+                    help="fused paths in the caller's numbering (no patch-order numbering)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the path is a single-GPU kernel set "
@@ -314,7 +318,8 @@ def main():
 
     cs = cylinder_case(args.nr, args.nt, mach=0.8, re=1000.0)
     prec = PRECISION[args.precision]
-    r = run_path(cs, prec, args.steps, args.warmup, breakdown=True)
+    rn = not args.no_renumber
+    r = run_path(cs, prec, args.steps, args.warmup, breakdown=True, renumber=rn)
     DTYPE = {0: "f64", 1: "f32-element/f64-accumulate", 2: "f64"}
     PATH = {0: "fused FP64", 1: "fused FP32-element", 2: "FP64 reference-order"}
     out = {
@@ -332,7 +337,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"cylinder O-grid Nr={args.nr} Ntheta={args.nt} ({cs.npoin} nodes, "
-                        f"{cs.nelem} triangles), M=0.8, Re=1000, fixed mesh, {PATH[prec]} path",
+                        f"{cs.nelem} triangles), M=0.8, Re=1000, fixed mesh, {PATH[prec]} path"
+                        + (", patch-order numbering inside the session" if rn and prec in (0, 1)
+                           else ""),
             "nelem": cs.nelem, "npoin": cs.npoin, "calcrhs_per_step": 4,
         },
         "roofline": {
@@ -342,8 +349,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": r["achieved"] / HBM_PEAK_GBS,
+            # measured with the default numbering of each path (profiles/)
             "traffic": (read_traffic(TRAFFIC[prec] + "_cylinder1024x2048_traffic.txt")
-                        if (args.nr, args.nt) == (1024, 2048) else None),
+                        if (args.nr, args.nt) == (1024, 2048) and (rn or prec == 2) else None),
             "algorithmic_bytes_per_launch": element_bytes(cs.nelem, cs.npoin, prec),
             "avg_launch_us": r["avg_elem_us"],
         },
@@ -356,7 +364,7 @@ def main():
         out["variants"] = {}
         for name, p in PRECISION.items():
             if p != prec:
-                v = run_path(cs, p, 20, 20)
+                v = run_path(cs, p, 20, 20, renumber=rn)
                 out["variants"][name] = {
                     "path": PATH[p], "value": v["value"], "ms_per_step": v["ms_per_step"],
                     "kernel": KERNEL[p], "avg_launch_us": v["avg_elem_us"],
