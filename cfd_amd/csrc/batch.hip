@@ -48,8 +48,10 @@ struct cfd_batch {
     DBuf<Consts> cases;  // per case
     DBuf<double> ctl, partials, hmin;
     // host copies for re-building patches (autotuning)
-    std::vector<std::vector<int>> inpoel_h;
+    std::vector<std::vector<int>> inpoel_h;   // per case, in patch-order numbering
     std::vector<std::vector<double>> X_h, Y_h;
+    // per case: the caller's node / element at each patch-order position
+    std::vector<std::vector<int>> node_old, elem_old;
     // graphs: one recorded step per U buffer parity
     bool use_graph = true;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -266,6 +268,25 @@ int cfd_batch_create(cfd_batch** out, int ncases, const cfd_batch_case* cases, i
         B->np = (int)np;
         B->ne = (int)ne;
         B->c = B->cc[0];
+        const int pe = patch_elems > 0 ? patch_elems : (precision == 1 ? kPatchElems : kPatchElems64);
+        const int blk = patch_block > 0 ? patch_block : 256;
+        CFD_ARG(pe >= 1 && pe <= 682, "patch_elems must be 1..682");
+        CFD_ARG(blk % 64 == 0 && blk >= 64 && blk <= 512,
+                "patch_block must be a multiple of 64 from 64 to 512");
+        // each case numbered in patch order inside (elements in patch order,
+        // each patch's nodes in turn): the same results bit for bit as the
+        // caller's numbering on these fixed meshes, with contiguous patch nodes
+        B->node_old.resize(ncases);
+        B->elem_old.resize(ncases);
+        std::vector<std::vector<int>> node_new(ncases);
+        for (int k = 0; k < ncases; ++k) {
+            const cfd_batch_case& q = cases[k];
+            for (long long i = 0; i < 3ll * q.nelem; ++i)
+                CFD_CHECK(q.inpoel[i] >= 1 && q.inpoel[i] <= q.npoin, "mesh: node id out of range");
+            patch_numbering(q.inpoel, q.X, q.Y, q.nelem, q.npoin, pe, B->elem_old[k], B->node_old[k]);
+            node_new[k].assign(q.npoin, 0);
+            for (int i = 0; i < q.npoin; ++i) node_new[k][B->node_old[k][i]] = i;
+        }
         // the union mesh, coordinates and boundary lists (ids shifted per case)
         std::vector<int> inp(3 * (size_t)ne);
         std::vector<double> X(np), Y(np);
@@ -277,20 +298,26 @@ int cfd_batch_create(cfd_batch** out, int ncases, const cfd_batch_case* cases, i
         for (int k = 0; k < ncases; ++k) {
             const cfd_batch_case& q = cases[k];
             const int N = B->nbase[k], E = B->ebase[k];
-            for (long long i = 0; i < 3ll * q.nelem; ++i) {
-                const int n = q.inpoel[i];
-                CFD_CHECK(n >= 1 && n <= q.npoin, "mesh: node id out of range");
-                inp[3 * (size_t)E + i] = n + N;
+            const std::vector<int>& nn = node_new[k];
+            const std::vector<int>& eo = B->elem_old[k];
+            std::vector<int>& lin = B->inpoel_h[k];
+            lin.resize(3 * (size_t)q.nelem);
+            for (int e = 0; e < q.nelem; ++e)
+                for (int j = 0; j < 3; ++j) {  // each element keeps its node order
+                    const int n = nn[q.inpoel[3 * (size_t)eo[e] + j] - 1] + 1;
+                    lin[3 * (size_t)e + j] = n;
+                    inp[3 * ((size_t)E + e) + j] = n + N;
+                }
+            B->X_h[k].resize(q.npoin);
+            B->Y_h[k].resize(q.npoin);
+            for (int i = 0; i < q.npoin; ++i) {
+                B->X_h[k][i] = X[N + i] = q.X[B->node_old[k][i]];
+                B->Y_h[k][i] = Y[N + i] = q.Y[B->node_old[k][i]];
             }
-            B->inpoel_h[k].assign(q.inpoel, q.inpoel + 3 * (size_t)q.nelem);
-            B->X_h[k].assign(q.X, q.X + q.npoin);
-            B->Y_h[k].assign(q.Y, q.Y + q.npoin);
-            std::copy(q.X, q.X + q.npoin, X.begin() + N);
-            std::copy(q.Y, q.Y + q.npoin, Y.begin() + N);
             const cfd_bc_lists& b = q.bc;
             auto shift = [&](int n) {
                 CFD_CHECK(n >= 1 && n <= q.npoin, "boundary list node id out of range");
-                return n + N;
+                return nn[n - 1] + 1 + N;
             };
             for (int i = 0; i < b.nfixv; ++i) {
                 fv.push_back(shift(b.ifixv_node[i]));
@@ -321,6 +348,13 @@ int cfd_batch_create(cfd_batch** out, int ncases, const cfd_batch_case* cases, i
         ub.nwall = (int)wall.size() / 2;
         ub.wall = wall.data();
         build_tables(B->t, inp.data(), B->ne, B->np, s);
+        {  // per-node sums (MASAS) in each case's original element order
+            std::vector<int> key(ne);
+            for (int k = 0; k < ncases; ++k)
+                for (int e = 0; e < cases[k].nelem; ++e)
+                    key[B->ebase[k] + e] = B->ebase[k] + B->elem_old[k][e];
+            reorder_esup(B->t, key, s);
+        }
         B->bc.upload(build_bc_host(B->np, &ub), B->np, &ub, s);
         // state
         for (auto* b : {&B->Ub[0], &B->Ub[1], &B->RHS}) b->alloc(4 * (size_t)np);
@@ -373,11 +407,6 @@ int cfd_batch_create(cfd_batch** out, int ncases, const cfd_batch_case* cases, i
         launch_deriv(B->t, B->X.p, B->Y.p, B->AREA.p, B->DNX.p, B->DNY.p, B->HH.p, B->HHX.p,
                      B->HHY.p, B->hmin.p, B->partials.p, s);
         launch_masas(B->t, B->AREA.p, B->M.p, s);
-        const int pe = patch_elems > 0 ? patch_elems : (precision == 1 ? kPatchElems : kPatchElems64);
-        const int blk = patch_block > 0 ? patch_block : 256;
-        CFD_ARG(pe <= 682, "patch_elems must be at most 682");
-        CFD_ARG(blk % 64 == 0 && blk >= 64 && blk <= 512,
-                "patch_block must be a multiple of 64 from 64 to 512");
         B->build(pe, blk);
         CFD_ARG(patch_lds_fits(B->patches.max_nodes, pe,
                                precision == 1 ? sizeof(float4) : sizeof(double4)),
@@ -451,9 +480,14 @@ int cfd_batch_get_field(cfd_batch* b, int icase, int field, double* host) {
         }
         const size_t o = node ? b->nbase[icase] : b->ebase[icase];
         const size_t n = node ? b->nbase[icase + 1] - o : b->ebase[icase + 1] - o;
-        CFD_HIP(hipMemcpyAsync(host, buf->p + w * o, sizeof(double) * w * n, hipMemcpyDeviceToHost,
-                               b->s));
+        std::vector<double> tmp(w * n);
+        CFD_HIP(hipMemcpyAsync(tmp.data(), buf->p + w * o, sizeof(double) * w * n,
+                               hipMemcpyDeviceToHost, b->s));
         CFD_HIP(hipStreamSynchronize(b->s));
+        // back to the caller's numbering of the case
+        const std::vector<int>& old = node ? b->node_old[icase] : b->elem_old[icase];
+        for (size_t i = 0; i < n; ++i)
+            for (int c = 0; c < w; ++c) host[w * (size_t)old[i] + c] = tmp[w * i + c];
     })
 }
 
