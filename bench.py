@@ -163,7 +163,7 @@ def batch_sweep(steps: int = 100, warmup: int = 5, ncases: int = 256) -> dict:
     t0 = time.perf_counter()
     B = Batch(cases, precision=1)
     t_create = time.perf_counter() - t0
-    cand = [(512, 256), (640, 320), (384, 192), (640, 256), (512, 512)]
+    cand = [(512, 256), (384, 256), (640, 320), (384, 192), (640, 256), (512, 512)]
     t0 = time.perf_counter()
     best, times = B.autotune(cand, steps=3)
     t_tune = time.perf_counter() - t0
