@@ -121,3 +121,49 @@ def test_full_size_fp32_residual(big_cyl):
     # (measured 2.9e-7; the other components about 1e-10)
     assert np.all(relnorm_cols(got["U"], ref["U"]) < 1e-6)
     assert got["DTMIN"] == ref["DTMIN"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_full_size_renumbered_session_bitwise(big_cyl, precision):
+    """The bench's configuration (fused path, patch-order numbering inside the
+    session) at full size: two steps from the oracle's state, bit for bit the
+    session in the caller's numbering."""
+    from cfd_amd.solver import Session
+    cs, o0 = big_cyl
+    st = o0.state()
+    out = []
+    for rn in (0, 1):
+        S = Session(cs, precision=precision, renumber=rn)
+        S.set_state(st)
+        S.set_scalars(DISN2=o0.disn2)
+        S.step(2)
+        out.append({k: S.get(k) for k in ("U", "T", "RHS", "DTL", "SHOC")} | S.scalars())
+        S.close()
+    a, b = out
+    for k in ("U", "T", "RHS", "DTL", "SHOC"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["DTMIN"] == b["DTMIN"] and a["FX1"] == b["FX1"]
+
+
+def test_full_size_batch_sweep():
+    """The 256-case sweep (10.8M nodes, 21.5M triangles) as the bench runs it:
+    every case finite after 20 steps with its own positive time step, and two
+    of its cases bit for bit against their single-case sessions."""
+    from cfd_amd.batch import Batch
+    from cfd_amd.mesh import sweep_cases
+    from cfd_amd.solver import Session
+    cases = sweep_cases(256)
+    B = Batch(cases, precision=1)
+    B.step(20)
+    B.sync()
+    dts = [B.scalars(k)["DTMIN"] for k in range(len(cases))]
+    assert all(np.isfinite(d) and d > 0 for d in dts)
+    for k in range(len(cases)):
+        assert np.isfinite(B.get(k, "U")).all(), k
+    for k in (0, 137):
+        S = Session(cases[k], precision=1)
+        S.step(20)
+        assert np.array_equal(B.get(k, "U"), S.get("U")), k
+        assert B.scalars(k)["DTMIN"] == S.scalars()["DTMIN"], k
+        S.close()
+    B.close()
