@@ -58,7 +58,19 @@ __device__ inline void block_partial(double v, Op op, double* partials) {
 template <class Op>
 __device__ inline double combine_partials(const double* partials, int n, double init, Op op) {
     double acc = init;
-    for (int b = threadIdx.x; b < n; b += blockDim.x) acc = op(acc, partials[b]);
+    // each thread folds partials threadIdx.x, + blockDim.x, ... in that order;
+    // sixteen at a time are loaded before they are folded, so that large grids
+    // (hundreds of partials per thread) keep many loads in flight
+    const int stride = (int)blockDim.x;
+    int b = (int)threadIdx.x;
+    for (; b + 15 * stride < n; b += 16 * stride) {
+        double v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = partials[b + q * stride];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc = op(acc, v[q]);
+    }
+    for (; b < n; b += stride) acc = op(acc, partials[b]);
     return block_reduce(acc, op);
 }
 
