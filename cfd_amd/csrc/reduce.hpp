@@ -59,8 +59,8 @@ template <class Op>
 __device__ inline double combine_partials(const double* partials, int n, double init, Op op) {
     double acc = init;
     // each thread folds partials threadIdx.x, + blockDim.x, ... in that order;
-    // sixteen at a time are loaded before they are folded, so that large grids
-    // (hundreds of partials per thread) keep many loads in flight
+    // sixteen (then eight, four) at a time are loaded before they are folded,
+    // so that large grids keep many loads in flight
     const int stride = (int)blockDim.x;
     int b = (int)threadIdx.x;
     for (; b + 15 * stride < n; b += 16 * stride) {
@@ -69,6 +69,22 @@ __device__ inline double combine_partials(const double* partials, int n, double 
         for (int q = 0; q < 16; ++q) v[q] = partials[b + q * stride];
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc = op(acc, v[q]);
+    }
+    if (b + 7 * stride < n) {  // then eight
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = partials[b + q * stride];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc = op(acc, v[q]);
+        b += 8 * stride;
+    }
+    if (b + 3 * stride < n) {  // then four
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = partials[b + q * stride];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc = op(acc, v[q]);
+        b += 4 * stride;
     }
     for (; b < n; b += stride) acc = op(acc, partials[b]);
     return block_reduce(acc, op);
