@@ -90,6 +90,9 @@ def test_full_size_cylinder_step(big_cyl, precision):
     print(f"precision {precision}: U rel err per component {rel}")
     assert got["DTMIN"] == ref["DTMIN"]
     assert np.all(rel < TOL), rel
+    rhs = relnorm_cols(got["RHS"], ref["RHS"])
+    print(f"precision {precision}: RHS rel err per component {rhs}")
+    assert np.all(rhs < TOL), rhs
     for f in ("T", "VEL_X", "VEL_Y", "P"):
         assert relnorm(got[f], ref[f]) < TOL, f
     # the state moved: the comparison is not of two copies of the input
@@ -104,6 +107,9 @@ def test_full_size_pitching_ellipse_step(big_ell, precision):
     print(f"precision {precision}: U rel err per component {rel}")
     assert got["DTMIN"] == ref["DTMIN"]
     assert np.all(rel < TOL), rel
+    rhs = relnorm_cols(got["RHS"], ref["RHS"])
+    print(f"precision {precision}: RHS rel err per component {rhs}")
+    assert np.all(rhs < TOL), rhs
     for f in ("T", "VEL_X", "VEL_Y", "P", "X", "Y", "M"):
         assert relnorm(got[f], ref[f]) < TOL, f
     # both mesh solves stop under the reference's absolute rule
