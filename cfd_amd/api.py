@@ -162,6 +162,25 @@ class MeshContext:
         bc = bc_lists(lists, keep)
         check(lib().cfd_normales(self._h, C.byref(bc), dptr(_f64(X)), dptr(_f64(Y))))
 
+    def normalvel(self, VEL_X, VEL_Y, W_X, W_Y):
+        """subrutinas.f90:66-85 with the normals of the last normales():
+        returns the projected (VEL_X, VEL_Y)"""
+        vx, vy = _f64(VEL_X).copy(), _f64(VEL_Y).copy()
+        check(lib().cfd_normalvel(self._h, dptr(vx), dptr(vy), dptr(_f64(W_X)), dptr(_f64(W_Y))))
+        return vx, vy
+
+    def force_visc(self, iset, X, Y, P, T, VEL_X, VEL_Y, dNx, dNy, t_inf):
+        """ns2DComp.ALE.f90:819-893 for one set of edges (n1, n2, elem):
+        (F_VX, F_VY)"""
+        ist = _i32(iset).reshape(-1)
+        fvx, fvy = C.c_double(), C.c_double()
+        check(lib().cfd_force_visc(self._h, ist.size // 3, iptr(ist) if ist.size else None,
+                                   dptr(_f64(X)), dptr(_f64(Y)), dptr(_f64(P)), dptr(_f64(T)),
+                                   dptr(_f64(VEL_X)), dptr(_f64(VEL_Y)), dptr(_f64(dNx)),
+                                   dptr(_f64(dNy)), C.c_double(t_inf), C.byref(fvx),
+                                   C.byref(fvy)))
+        return fvx.value, fvy.value
+
     def RK(self, consts: Consts, nrk, dtmin, dtl, st: dict):
         """subrutinas.f90:645-849 on a state dict (same keys as the oracle);
         updates U1, RHS, T, VEL_X, VEL_Y, RHO, E, P, RMACH, SHOC, T_SUGN1..3"""
@@ -211,6 +230,23 @@ def biCG(spMtx, spIdx, spRowptr, diagMtx, x, b, x_fix, x_fixIdx):
                          dptr(_f64(diagMtx)), dptr(x), dptr(_f64(b)), dptr(xf) if xf.size else None,
                          iptr(fi) if fi.size else None, int(fi.size), C.byref(it), C.byref(rz)))
     return it.value, rz.value
+
+
+def write_restart(path: str, iter_: int, time_: float, U, T, GAMM) -> None:
+    """PRINTREST (ns2DComp.ALE.f90:898-917): the reference's .RST file."""
+    U = _f64(U).reshape(-1, 4)
+    check(lib().cfd_write_restart(str(path).encode(), int(iter_), C.c_double(time_), U.shape[0],
+                                  dptr(U), dptr(_f64(T)), dptr(_f64(GAMM))))
+
+
+def read_restart(path: str, npoin: int):
+    """RESTART's restart branch (ns2DComp.ALE.f90:423-433) -> (ITER, TIME, U,
+    T, GAMM); the reference reads ITER and TIME into locals and drops them."""
+    U, T, G = np.empty((npoin, 4)), np.empty(npoin), np.empty(npoin)
+    it, tm = C.c_int(), C.c_double()
+    check(lib().cfd_read_restart(str(path).encode(), int(npoin), C.byref(it), C.byref(tm),
+                                 dptr(U), dptr(T), dptr(G)))
+    return it.value, tm.value, U, T, G
 
 
 def biCG_ex(spMtx, spIdx, spRowptr, diagMtx, x, b, x_fix, x_fixIdx, tol=1e-10, maxit=1000, rel=0):

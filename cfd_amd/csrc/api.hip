@@ -329,6 +329,52 @@ int cfd_normales(cfd_mesh* m, const cfd_bc_lists* bc, const double* X, const dou
     })
 }
 
+int cfd_normalvel(cfd_mesh* m, double* VEL_X, double* VEL_Y, const double* W_X, const double* W_Y) {
+    CFD_TRY({
+        CFD_ARG(m && VEL_X && VEL_Y && W_X && W_Y, "null argument");
+        CFD_CHECK(m->bc, "state: cfd_normales must run before cfd_normalvel");
+        const int np = m->t.npoin;
+        double* vx = m->put("VX", VEL_X, np);
+        double* vy = m->put("VY", VEL_Y, np);
+        double* wx = m->put("WX", W_X, np);
+        double* wy = m->put("WY", W_Y, np);
+        launch_normalvel(m->bc->tab, vx, vy, wx, wy, m->s);
+        m->get(VEL_X, (const double*)vx, np);
+        m->get(VEL_Y, (const double*)vy, np);
+        m->sync();
+    })
+}
+
+int cfd_force_visc(cfd_mesh* m, int nedge, const int* iset, const double* X, const double* Y,
+                   const double* P, const double* T, const double* VEL_X, const double* VEL_Y,
+                   const double* dNx, const double* dNy, double t_inf, double* f_vx, double* f_vy) {
+    CFD_TRY({
+        CFD_ARG(m && f_vx && f_vy && nedge >= 0 && (nedge == 0 || iset), "null argument");
+        const int np = m->t.npoin, ne = m->t.nelem;
+        for (int q = 0; q < nedge; ++q) {
+            CFD_ARG(iset[3 * q] >= 1 && iset[3 * q] <= np && iset[3 * q + 1] >= 1 &&
+                        iset[3 * q + 1] <= np, "iset node out of range");
+            CFD_ARG(iset[3 * q + 2] >= 1 && iset[3 * q + 2] <= ne, "iset element out of range");
+        }
+        *f_vx = 0.0;
+        *f_vy = 0.0;
+        if (nedge == 0) return CFD_OK;
+        const int* is = m->put("iset", iset, 3 * (size_t)nedge);
+        ViscIn v{m->t.inpoel.p, m->put("T", T, np), m->put("VX", VEL_X, np),
+                 m->put("VY", VEL_Y, np), m->put("dNx", dNx, 3 * (size_t)ne),
+                 m->put("dNy", dNy, 3 * (size_t)ne), t_inf};
+        double* dX = m->put("X", X, np);
+        double* dY = m->put("Y", Y, np);
+        double* dP = m->put("P", P, np);
+        launch_forces(nedge, is, dX, dY, dP, 0.0, 0.0, nullptr, &v, m->dscal.p, m->s);
+        double out[2];
+        m->get(out, (const double*)m->dscal.p, 2);
+        m->sync();
+        *f_vx = out[0];
+        *f_vy = out[1];
+    })
+}
+
 int cfd_rk(cfd_mesh* m, const cfd_consts* c, const cfd_bc_lists* bc, int nrk, double dtmin,
            const double* dtl, cfd_rk_nodes* nd, cfd_rk_elems* el) {
     CFD_TRY({
@@ -739,8 +785,10 @@ struct cfd_session {
                          d.pcg_rel, s);
         last_solves_trivial = pcg_polled_iterations(pcg) == 0.0 && pcg_polled_iterations(pcg_y) == 0.0;
         te(CFD_K_PCG);
-        pcg_store_result(pcg, scal(CFD_S_PCG_ITX), scal(CFD_S_PCG_RZX), scal(CFD_S_PCG_ITSUM), s);
-        pcg_store_result(pcg_y, scal(CFD_S_PCG_ITY), scal(CFD_S_PCG_RZY), scal(CFD_S_PCG_ITSUM), s);
+        pcg_store_result(pcg, scal(CFD_S_PCG_ITX), scal(CFD_S_PCG_RZX), scal(CFD_S_PCG_ITSUM), s,
+                         scal(CFD_S_PCG_UNCONV));
+        pcg_store_result(pcg_y, scal(CFD_S_PCG_ITY), scal(CFD_S_PCG_RZY), scal(CFD_S_PCG_ITSUM), s,
+                         scal(CFD_S_PCG_UNCONV));
         if (ws_on) ws.after_solves(q, s);
         k_move<<<grid_for(np), kBlock, 0, s>>>(X.p, X1.p, WX.p, XPOS.p, scal(CFD_S_DTMIN), np);
         k_move<<<grid_for(np), kBlock, 0, s>>>(Y.p, Y1.p, WY.p, YPOS.p, scal(CFD_S_DTMIN), np);
@@ -809,8 +857,16 @@ struct cfd_session {
             CFD_HIP(hipMemcpyAsync(WXO.p, WX.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
             CFD_HIP(hipMemcpyAsync(WYO.p, WY.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
         }
-        launch_forces(d.nsetedges, ISET.p, X.p, Y.p, P.p, d.xref, d.yref, scal(CFD_S_FX), s);
-        if (d.nmove == 0 && xpos_zero && move_normalized) {
+        // FORCE_VISC as the reference's print block calls it (ns2DComp.ALE.f90:
+        // 231-236): after the mesh move, with the step's velocities, pressure
+        // and temperature and the shape gradients of the start of the step;
+        // only when FMU /= 0.  When nothing moves it shares FORCES' launch.
+        const ViscIn visc{t.inpoel.p, T.p, VX.p, VY.p, DNX.p, DNY.p, c.t_inf};
+        const bool fvisc = c.fmu != 0.0;
+        const bool still = d.nmove == 0 && xpos_zero && move_normalized;
+        launch_forces(d.nsetedges, ISET.p, X.p, Y.p, P.p, d.xref, d.yref, scal(CFD_S_FX),
+                      fvisc && still ? &visc : nullptr, scal(CFD_S_FVX), s);
+        if (still) {
             // both solves return at once: nothing moves (see xpos_zero)
             k_put2<<<1, 1, 0, s>>>(scal(CFD_S_PCG_ITX), 0.0, scal(CFD_S_PCG_RZX), 0.0);
             k_put2<<<1, 1, 0, s>>>(scal(CFD_S_PCG_ITY), 0.0, scal(CFD_S_PCG_RZY), 0.0);
@@ -822,6 +878,9 @@ struct cfd_session {
                               DXPOS.p, DYPOS.p, s);
             mesh_solves();
             move_normalized = true;
+            if (fvisc)
+                launch_forces(d.nsetedges, ISET.p, X.p, Y.p, P.p, d.xref, d.yref, nullptr, &visc,
+                              scal(CFD_S_FVX), s);
         }
 
         if (d.moving == 1) {  // ns2DComp.ALE.f90:259-275

@@ -405,6 +405,10 @@ struct WarmStart {
     void after_solves(const WsSolve& q, hipStream_t s);   // training and history
 };
 
+// NORMALVEL (subrutinas.f90:66-85) on the BC table entries NORMALES marked:
+// VX, VY in place from W
+void launch_normalvel(const BcTable& bc, double* VX, double* VY, const double* WX,
+                      const double* WY, hipStream_t s);
 // NORMALES (subrutinas.f90:7-64) on the BC table entries that are wall nodes
 void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, const int* wnode_ptr,
                      const int* wnode_edges, const int* wall, const double* X, const double* Y,
@@ -469,9 +473,17 @@ void launch_step_ctl(const double* partials, const int* pb, int nb, double* ctl,
 void launch_dt_finish(double* DT, double* DTL, const double* ctl, const int* ecase, bool blend,
                       int nelem, hipStream_t s);
 
-// meshMove.f90 FORCES / TRANSF / the node updates of fluidStructure
+// meshMove.f90 FORCES (out3 = FX, FY, RM; skipped when null) and, when visc
+// is given, FORCE_VISC (ns2DComp.ALE.f90:819-893; out2 = F_VX, F_VY) in the
+// same launch; device scalars out
+struct ViscIn {
+    const int* inpoel;
+    const double *T, *VX, *VY, *dNx, *dNy;
+    double t_inf;
+};
 void launch_forces(int nedge, const int* iset, const double* X, const double* Y, const double* P,
-                   double xref, double yref, double* out3, hipStream_t s);
+                   double xref, double yref, double* out3, const ViscIn* visc, double* out2,
+                   hipStream_t s);
 // cs: device scalars cos(alpha), sin(alpha)
 void launch_transf(int nedge, const int* iset, const double* X, const double* Y, const double* cs,
                    double yposr, double xref, double yref, double* DXPOS, double* DYPOS,
@@ -522,8 +534,10 @@ void pcg_launch(PcgWork& w, int npoin, const double* A, const int* idx, const in
 // Waits for the stream and returns the last solve's outcome.
 PcgResult pcg_result(const PcgWork& w, hipStream_t s);
 // Copy the last solve's iteration count and final r.z to device scalars
-// (and add the count to *itsum when given).
-void pcg_store_result(const PcgWork& w, double* iters, double* rz, double* itsum, hipStream_t s);
+// (and add the count to *itsum when given, and 1 to *unconv when given and
+// the solve stopped at its iteration cap without meeting the tolerance).
+void pcg_store_result(const PcgWork& w, double* iters, double* rz, double* itsum, hipStream_t s,
+                      double* unconv = nullptr);
 // r.r of the first residual test for the guess x (fixed values applied to a
 // copy; x is not changed) into the device scalar rr_out
 void pcg_initial_rr(PcgWork& w, int npoin, const double* A, const int* idx, const int* rowptr,

@@ -385,10 +385,12 @@ k_fin_b2(PairSys s0, PairSys s1, int nb) {
 }
 
 __global__ void k_store_result(const double* __restrict__ scal, double* __restrict__ iters,
-                               double* __restrict__ rz, double* __restrict__ itsum) {
+                               double* __restrict__ rz, double* __restrict__ itsum,
+                               double* __restrict__ unconv) {
     *iters = scal[S_K];
     *rz = scal[S_EOLD];
     if (itsum) *itsum = *itsum + scal[S_K];
+    if (unconv && scal[S_CONV] == 0.0) *unconv = *unconv + 1.0;
 }
 
 __global__ void __launch_bounds__(256)
@@ -414,8 +416,9 @@ void pcg_initial_rr(PcgWork& w, int npoin, const double* A, const int* idx, cons
     CFD_HIP(hipGetLastError());
 }
 
-void pcg_store_result(const PcgWork& w, double* iters, double* rz, double* itsum, hipStream_t s) {
-    k_store_result<<<1, 1, 0, s>>>(w.scal.p, iters, rz, itsum);
+void pcg_store_result(const PcgWork& w, double* iters, double* rz, double* itsum, hipStream_t s,
+                      double* unconv) {
+    k_store_result<<<1, 1, 0, s>>>(w.scal.p, iters, rz, itsum, unconv);
     CFD_HIP(hipGetLastError());
 }
 

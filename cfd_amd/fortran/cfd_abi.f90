@@ -67,7 +67,8 @@ module cfd_abi
         cfd_s_bandera = 3, cfd_s_iter = 4, cfd_s_hmin = 5, cfd_s_fx = 6, cfd_s_fy = 7, &
         cfd_s_rm = 8, cfd_s_disn2 = 9, cfd_s_pcg_itx = 10, cfd_s_pcg_ity = 11, &
         cfd_s_pcg_rzx = 12, cfd_s_pcg_rzy = 13, cfd_s_ws_learned = 14, cfd_s_ws_solves = 15, &
-        cfd_s_ws_loss = 16, cfd_s_pcg_itsum = 17, cfd_nscalars = 18
+        cfd_s_ws_loss = 16, cfd_s_pcg_itsum = 17, cfd_s_fvx = 18, cfd_s_fvy = 19, &
+        cfd_s_pcg_unconv = 20, cfd_nscalars = 21
 
     ! RK's module arrays (subrutinas.f90:645-849), nodal and per element
     type, bind(c) :: cfd_rk_nodes
@@ -195,6 +196,27 @@ module cfd_abi
             type(c_ptr), value :: m
             type(cfd_bc_lists), intent(in) :: bc
             real(c_double), intent(in) :: x(*), y(*)
+        end function
+        ! Mnormales::normalvel (subrutinas.f90:66-85) with the normals of cfd_normales
+        integer(c_int) function cfd_normalvel(m, vel_x, vel_y, w_x, w_y) &
+                bind(c, name='cfd_normalvel')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(inout) :: vel_x(*), vel_y(*)
+            real(c_double), intent(in) :: w_x(*), w_y(*)
+        end function
+        ! FORCE_VISC (ns2DComp.ALE.f90:819-893) for one set of edges iset(3,nedge)
+        integer(c_int) function cfd_force_visc(m, nedge, iset, x, y, p, t, vel_x, vel_y, dnx, &
+                                               dny, t_inf, f_vx, f_vy) &
+                bind(c, name='cfd_force_visc')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            integer(c_int), value :: nedge
+            integer(c_int), intent(in) :: iset(*)
+            real(c_double), intent(in) :: x(*), y(*), p(*), t(*), vel_x(*), vel_y(*), dnx(*), &
+                                          dny(*)
+            real(c_double), value :: t_inf
+            real(c_double), intent(out) :: f_vx, f_vy
         end function
         ! RK (subrutinas.f90:645-849, NGAS = 0)
         integer(c_int) function cfd_rk(m, c, bc, nrk, dtmin, dtl, nodes, elems) &

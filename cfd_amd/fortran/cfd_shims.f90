@@ -6,7 +6,7 @@
 !   BiconjGrad), mLaplace.f90 (module Mlaplace), calcRHS.f90 (module
 !   calcRHS_mod), gcl.f90 (module gcl_mod), and from subrutinas.f90 the
 !   external subroutines deriv, MASAS, deltat, ESTAB, FUENTE and RK plus
-!   module Mnormales (normales).
+!   module Mnormales (normales, normalvel).
 ! The hidden inputs the reference reads by USE (InputData, MeshData,
 ! Mvariables, MVELOCIDADES, MVARIABGEN, MESTABILIZACION) are read the same
 ! way here, so this file compiles only together with the reference's own
@@ -177,6 +177,14 @@ contains
     subroutine normales
         call shim_bc_call(.true.)
     end subroutine normales
+
+    ! subrutinas.f90:66-85: the module velocities of the wall nodes the last
+    ! normales found keep their tangential part relative to the mesh velocity
+    ! (ADAMSB calls it; RK applies it inside the library)
+    subroutine normalvel
+        use MVELOCIDADES, only: VEL_X, VEL_Y, W_X, W_Y
+        call cfd_check(cfd_normalvel(shim_mesh_data(), VEL_X, VEL_Y, W_X, W_Y), 'normalvel')
+    end subroutine normalvel
 
     ! NORMALES, or RK's boundary lists only; the lists are copied into TARGET
     ! arrays so their addresses can be handed over

@@ -154,6 +154,11 @@ class Case:
     cte_in: float = 1.0
     itlocal: int = 0
     slip_body: bool = False      # body edges as slip walls (NORMALES) instead of no-slip
+    # pitching law of a moving body: 0 = the reference's pi/8 sin(10 t)
+    # (meshMove.f90:55-57,70); 1 = motion_amp sin(motion_omega t) (radians)
+    motion: int = 0
+    motion_amp: float = 0.0
+    motion_omega: float = 0.0
     # derived (filled by __post_init__) -------------------------------------
     fr: float = field(init=False)
     u_inf: float = field(init=False)

@@ -22,14 +22,16 @@ SCALAR_OF = {"DTMIN": "CFD_S_DTMIN", "TIME": "CFD_S_TIME", "DTMIN1": "CFD_S_DTMI
              "PCG_ITX": "CFD_S_PCG_ITX", "PCG_ITY": "CFD_S_PCG_ITY", "PCG_RZX": "CFD_S_PCG_RZX",
              "PCG_RZY": "CFD_S_PCG_RZY", "WS_LEARNED": "CFD_S_WS_LEARNED",
              "WS_SOLVES": "CFD_S_WS_SOLVES", "WS_LOSS": "CFD_S_WS_LOSS",
-             "PCG_ITSUM": "CFD_S_PCG_ITSUM"}
+             "PCG_ITSUM": "CFD_S_PCG_ITSUM", "FVX1": "CFD_S_FVX", "FVY1": "CFD_S_FVY",
+             "PCG_UNCONV": "CFD_S_PCG_UNCONV"}
 NSCALARS = SCALARS["CFD_NSCALARS"]
 WIDTH = {"U": 4, "U1": 4, "RHS": 4, "DNX": 3, "DNY": 3}
 
 
 class Session:
-    def __init__(self, cs, mesh: MeshContext | None = None, motion: int = 0,
-                 motion_amp: float = 0.0, motion_omega: float = 0.0, pcg_tol: float = 0.0,
+    def __init__(self, cs, mesh: MeshContext | None = None, motion: int | None = None,
+                 motion_amp: float | None = None, motion_omega: float | None = None,
+                 pcg_tol: float = 0.0,
                  pcg_maxit: int = 0, gcl: int = -1, precision: int = 0, init: bool = True,
                  pcg_rel: int = 0, warm_start: int = 0, patch_elems: int = 0,
                  patch_block: int = 0, renumber: int = 0):
@@ -55,7 +57,10 @@ class Session:
         d.ilaux = iptr(ilaux) if ilaux.size else None
         d.bc = bc_lists(L, self._keep)
         d.X, d.Y = dptr(X), dptr(Y)
-        d.motion, d.motion_amp, d.motion_omega = motion, motion_amp, motion_omega
+        # the pitching law: the case's unless given here
+        d.motion = int(cs.motion if motion is None else motion)
+        d.motion_amp = float(cs.motion_amp if motion_amp is None else motion_amp)
+        d.motion_omega = float(cs.motion_omega if motion_omega is None else motion_omega)
         d.pcg_tol, d.pcg_maxit, d.gcl, d.precision = pcg_tol, pcg_maxit, gcl, precision
         d.pcg_rel = pcg_rel
         d.warm_start = warm_start
@@ -134,6 +139,15 @@ class Session:
         sc = {k: st[k] for k in ("DTMIN", "TIME", "DTMIN1", "BANDERA", "ITER", "DISN2") if k in st}
         if sc:
             self.set_scalars(**sc)
+
+    # ---- restart files (ns2DComp.ALE.f90:394-435, 898-917) -------------------------
+    def write_restart(self, path: str) -> None:
+        check(lib().cfd_session_write_restart(self._h, str(path).encode()))
+
+    def read_restart(self, path: str, keep_clock: bool = False) -> None:
+        """U and T from the file (U1 and the primitives follow); the step
+        control restarts; TIME and ITER from the file only with keep_clock."""
+        check(lib().cfd_session_read_restart(self._h, str(path).encode(), int(keep_clock)))
 
     # ---- timing --------------------------------------------------------------------
     def timing(self, enable: bool = True, slots=None) -> None:

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CFD_ABI_VERSION 4
+#define CFD_ABI_VERSION 5
 
 enum {
     CFD_OK = 0,
@@ -118,6 +118,12 @@ typedef struct {
 /* Mnormales::normales (subrutinas.f90:7-64): wall normals from X, Y; kept in
  * the mesh context for cfd_rk (the reference keeps them module-private). */
 int cfd_normales(cfd_mesh *m, const cfd_bc_lists *bc, const double *X, const double *Y);
+/* Mnormales::normalvel (subrutinas.f90:66-85): on the wall nodes the last
+ * cfd_normales call found, VEL_X/VEL_Y (npoin, in place) keep only their
+ * tangential part relative to the mesh velocity W_X/W_Y, plus W.  Needs a
+ * prior cfd_normales on this mesh (CFD_ERR_STATE otherwise). */
+int cfd_normalvel(cfd_mesh *m, double *VEL_X, double *VEL_Y, const double *W_X,
+                  const double *W_Y);
 
 /* RK (subrutinas.f90:645-849, NGAS=0): the reference passes these through
  * modules; here every array is explicit (host, node-major). */
@@ -131,6 +137,17 @@ typedef struct {
 } cfd_rk_elems;
 int cfd_rk(cfd_mesh *m, const cfd_consts *c, const cfd_bc_lists *bc, int nrk, double dtmin,
            const double *dtl, cfd_rk_nodes *nodes, cfd_rk_elems *elems);
+
+/* FORCE_VISC (ns2DComp.ALE.f90:819-893) for one force set: the traction of
+ * the full stress tensor (element-average pressure and the set element's P1
+ * velocity gradients, Sutherland viscosity at the edge-average temperature)
+ * on each edge (n1, n2, elem) of iset(3,nedge), times its length, summed in
+ * edge order into *f_vx, *f_vy.  The per-edge skin friction the reference
+ * writes to SKIN.DAT is not produced. */
+int cfd_force_visc(cfd_mesh *m, int nedge, const int *iset, const double *X, const double *Y,
+                   const double *P, const double *T, const double *VEL_X, const double *VEL_Y,
+                   const double *dNx, const double *dNy, double t_inf, double *f_vx,
+                   double *f_vy);
 
 /* BiconjGrad::biCG (biconjGrad.f90:8-62): Jacobi-PCG on a CSR matrix with
  * Dirichlet nodes; x is the initial guess in, the solution out.  Extensions
@@ -246,6 +263,10 @@ enum {
     CFD_S_WS_SOLVES,   /* mesh solves with a learned candidate so far        */
     CFD_S_WS_LOSS,     /* the MLP's last training loss                        */
     CFD_S_PCG_ITSUM,   /* mesh-solve iterations summed over both axes so far  */
+    CFD_S_FVX,         /* FORCE_VISC F_VX(1), F_VY(1) (ns2DComp.ALE.f90:     */
+    CFD_S_FVY,         /*   819-893) after the step's mesh move; 0 if FMU = 0 */
+    CFD_S_PCG_UNCONV,  /* mesh solves so far that stopped at the iteration cap
+                          without meeting the tolerance                      */
     CFD_NSCALARS
 };
 /* kernel timing slots (HIP events on the session stream) */
@@ -270,6 +291,27 @@ int cfd_session_sync(cfd_session *s);
 int cfd_session_field_size(cfd_session *s, int field, long long *count);
 int cfd_session_get_field(cfd_session *s, int field, double *host);
 int cfd_session_set_field(cfd_session *s, int field, const double *host);
+/* Restart files (ns2DComp.ALE.f90:394-435 RESTART, :898-917 PRINTREST): a
+ * Fortran unformatted sequential file, each record framed by its byte length
+ * (4-byte integer) before and after; record 0 holds ITER (int32) and TIME,
+ * then one record per node U(1:4,i), T(i), GAMM(i).  Host-only: no device is
+ * used.  cfd_read_restart reads the first npoin node records (more are
+ * ignored, fewer or misframed ones are an error) and returns ITER and TIME
+ * too, which the reference's RESTART reads and discards. */
+int cfd_write_restart(const char *path, int iter, double time, int npoin, const double *U,
+                      const double *T, const double *GAMM);
+int cfd_read_restart(const char *path, int npoin, int *iter, double *time, double *U,
+                     double *T, double *GAMM);
+/* A session's U, T, GAMM and its ITER, TIME to a restart file (the caller's
+ * numbering), and back: U and T from the file, GAMM left at GAMA (the main
+ * program replaces the loaded GAMM by GAMA when NGAS /= 1,
+ * ns2DComp.ALE.f90:78), and, beyond
+ * the reference (which leaves them unset), U1 = U and the primitives RHO,
+ * VEL_X, VEL_Y, E of U.  The step control restarts (DTMIN, DTMIN1 = 0,
+ * BANDERA = 1); keep_clock = 0 sets TIME = ITER = 0 as the main program does
+ * (:107-110), 1 keeps the file's. */
+int cfd_session_write_restart(cfd_session *s, const char *path);
+int cfd_session_read_restart(cfd_session *s, const char *path, int keep_clock);
 /* the CFD_S_* scalars; get waits for queued steps */
 int cfd_session_get_scalars(cfd_session *s, double *scalars /* CFD_NSCALARS */);
 int cfd_session_set_scalars(cfd_session *s, const double *scalars);

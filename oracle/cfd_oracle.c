@@ -397,6 +397,20 @@ void orc_fuente(int nelem, double *rhs, const double *U, const double *W_X,
 /* ------------------------------------------------------------------ */
 /* subrutinas.f90:645-849 RK (NGAS = 0 branch)                           */
 /* ------------------------------------------------------------------ */
+/* subrutinas.f90:66-85 NORMALVEL on the nnorm nodes NORMALES listed */
+void orc_normalvel(int nnorm, const int *n_ipoin, const double *n_x, const double *n_y,
+                   double *VEL_X, double *VEL_Y, const double *W_X, const double *W_Y)
+{
+    for (int i = 0; i < nnorm; ++i) {
+        int ip = n_ipoin[i] - 1;
+        double vx = VEL_X[ip], vy = VEL_Y[ip];
+        double wx = W_X[ip], wy = W_Y[ip];
+        double pp = -n_y[i] * (vx - wx) + n_x[i] * (vy - wy);
+        VEL_X[ip] = -n_y[i] * pp + wx;
+        VEL_Y[ip] = n_x[i] * pp + wy;
+    }
+}
+
 void orc_rk(const orc_consts *c, int nelem, int npoin, int nrk, double dtmin,
             const double *dtl, orc_nodes *nd, orc_elems *el, const orc_bcs *bc)
 {
@@ -435,14 +449,8 @@ void orc_rk(const orc_consts *c, int nelem, int npoin, int nrk, double dtmin,
             nd->VEL_Y[j] = bc->rfixv_valuey[i];
         }
         /* NORMALVEL :66-85 */
-        for (int i = 0; i < bc->nnorm; ++i) {
-            int ip = bc->n_ipoin[i] - 1;
-            double vx = nd->VEL_X[ip], vy = nd->VEL_Y[ip];
-            double wx = nd->W_X[ip], wy = nd->W_Y[ip];
-            double pp = -bc->n_y[i] * (vx - wx) + bc->n_x[i] * (vy - wy);
-            nd->VEL_X[ip] = -bc->n_y[i] * pp + wx;
-            nd->VEL_Y[ip] = bc->n_x[i] * pp + wy;
-        }
+        orc_normalvel(bc->nnorm, bc->n_ipoin, bc->n_x, bc->n_y, nd->VEL_X, nd->VEL_Y, nd->W_X,
+                      nd->W_Y);
         /* FIX :618-643 */
         for (int i = 0; i < bc->nfixrho; ++i)
             nd->RHO[bc->ifixrho_node[i] - 1] = bc->rfixrho_value[i];
@@ -633,6 +641,54 @@ void orc_forces(int nedge, const int *iset, const double *X, const double *Y,
         RM = RM + dfy * (xc - xref) - dfx * (yc - yref);
     }
     *fx = FX; *fy = FY; *rm = RM;
+}
+
+/* ------------------------------------------------------------------ */
+/* ns2DComp.ALE.f90:819-893 FORCE_VISC (single set): the traction of the  */
+/* full stress tensor, -p I + viscous, with the element-average pressure   */
+/* and the set element's P1 velocity gradients, Sutherland mu at the edge  */
+/* average temperature, times the edge length, summed in edge order.  The  */
+/* edge normal (RLX, RLY) = (Y2-Y1, -(X2-X1)) / |edge| is that of FORCES.  */
+/* SKIN.DAT (per-edge skin friction written to a file) is not restated.    */
+/* ------------------------------------------------------------------ */
+void orc_force_visc(int nedge, const int *iset, const int *inpoel, const double *X,
+                    const double *Y, const double *P, const double *T, const double *VEL_X,
+                    const double *VEL_Y, const double *dNx, const double *dNy, double t_inf,
+                    double *f_vx, double *f_vy)
+{
+    double FVX = 0.0, FVY = 0.0;
+    for (int ii = 0; ii < nedge; ++ii) {
+        int nn1 = iset[3 * ii] - 1, nn2 = iset[3 * ii + 1] - 1, ie = iset[3 * ii + 2] - 1;
+        double temp = (T[nn1] + T[nn2]) / 2.0;           /* :844 */
+        double smu = 110.0;
+        double fmu = 0.017 * pow(temp / t_inf, 1.5) * (t_inf + smu) / (temp + smu);
+        double rly = -(X[nn2] - X[nn1]);                 /* :849-851 */
+        double rlx = Y[nn2] - Y[nn1];
+        double rmod = sqrt(rlx * rlx + rly * rly);
+        rlx = rlx / rmod;
+        rly = rly / rmod;
+        double dux = 0.0, duy = 0.0, dvx = 0.0, dvy = 0.0, press = 0.0;
+        for (int jj = 0; jj < 3; ++jj) {                 /* :860-869 */
+            int nn = inpoel[3 * ie + jj] - 1;
+            double nx = dNx[3 * ie + jj], ny = dNy[3 * ie + jj];
+            dux = dux + nx * VEL_X[nn];
+            duy = duy + ny * VEL_X[nn];
+            dvx = dvx + nx * VEL_Y[nn];
+            dvy = dvy + ny * VEL_Y[nn];
+            press = press + P[nn];
+        }
+        press = press / 3.0;
+        double txx = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dux);   /* :873-877 */
+        double txy = fmu * (duy + dvx);
+        double tyx = txy;
+        double tyy = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dvy);
+        double ttx = txx * rlx + txy * rly;              /* :880-881 */
+        double tty = tyx * rlx + tyy * rly;
+        FVX = FVX + ttx * rmod;                          /* :885-886 */
+        FVY = FVY + tty * rmod;
+    }
+    *f_vx = FVX;
+    *f_vy = FVY;
 }
 
 void orc_transf(int nedge, const int *iset, const double *X, const double *Y,

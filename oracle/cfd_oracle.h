@@ -98,6 +98,12 @@ int orc_bicg(int npoin, const double *spMtx, const int *spIdx, const int *spRowp
 void orc_forces(int nedge, const int *iset /* (nedge,3): n1 n2 elem */,
                 const double *X, const double *Y, const double *P, double xref,
                 double yref, double *fx, double *fy, double *rm);
+void orc_normalvel(int nnorm, const int *n_ipoin, const double *n_x, const double *n_y,
+                   double *VEL_X, double *VEL_Y, const double *W_X, const double *W_Y);
+void orc_force_visc(int nedge, const int *iset, const int *inpoel, const double *X,
+                    const double *Y, const double *P, const double *T, const double *VEL_X,
+                    const double *VEL_Y, const double *dNx, const double *dNy, double t_inf,
+                    double *f_vx, double *f_vy);
 void orc_transf(int nedge, const int *iset, const double *X, const double *Y,
                 double alpha, double yposr, double xref, double yref,
                 double *DXPOS, double *DYPOS);

@@ -83,6 +83,9 @@ def lib():
         L.orc_bicg.restype = _i
         L.orc_forces.argtypes = [_i, _ip, _dp, _dp, _dp, _d, _d, _pd, _pd, _pd]
         L.orc_transf.argtypes = [_i, _ip, _dp, _dp, _d, _d, _d, _d, _dp, _dp]
+        L.orc_normalvel.argtypes = [_i, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_force_visc.argtypes = [_i, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _d,
+                                     _pd, _pd]
         L.orc_set_threads.argtypes = [_i]
         L.orc_get_threads.restype = _i
         L.orc_set_threads(1)
@@ -204,6 +207,7 @@ class OracleSolver:
         self.bandera = 1
         self.disn2 = 0.0          # MeshMove DISN(2), previous pitch angle
         self.fx = self.fy = self.rm = 0.0
+        self.fvx = self.fvy = 0.0
         self.last_pcg_iters = (0, 0)
 
     # ---- geometry ---------------------------------------------------------
@@ -303,15 +307,42 @@ class OracleSolver:
                          C.byref(fx), C.byref(fy), C.byref(rm))
         return fx.value, fy.value, rm.value
 
+    def normalvel(self, vx, vy, wx, wy):
+        """NORMALVEL (subrutinas.f90:66-85) with the normals of the last
+        normales(): the projected copies of vx, vy"""
+        n_ip, nx, ny = self.norm
+        vx, vy = np.array(vx, np.float64), np.array(vy, np.float64)
+        lib().orc_normalvel(n_ip.size, np.ascontiguousarray(n_ip, np.int32), nx, ny, vx, vy,
+                            np.ascontiguousarray(wx, np.float64), np.ascontiguousarray(wy, np.float64))
+        return vx, vy
+
+    def force_visc(self):
+        """FORCE_VISC (ns2DComp.ALE.f90:819-893) for force set 1: F_VX, F_VY."""
+        s = self.s
+        iset = np.ascontiguousarray(self.lists["iset"], np.int32)
+        fvx, fvy = C.c_double(), C.c_double()
+        lib().orc_force_visc(iset.shape[0], iset, self.inpoel, s["X"], s["Y"], s["P"], s["T"],
+                             s["VEL_X"], s["VEL_Y"], s["DNX"], s["DNY"], self.cs.t_inf,
+                             C.byref(fvx), C.byref(fvy))
+        return fvx.value, fvy.value
+
+    def pitch_angle(self, time):
+        """The body's pitch angle at `time`: the reference's law
+        pi/8 sin(10 t) (meshMove.f90:55-57,70), or with cs.motion == 1 the
+        parameterised law motion_amp sin(motion_omega t)."""
+        cs = self.cs
+        if getattr(cs, "motion", 0) == 1:
+            return cs.motion_amp * math.sin(cs.motion_omega * time)
+        return math.acos(-1.0) / 8.0 * math.sin(10.0 * time)
+
     def fluid_structure(self, dtmin, time):
         """meshMove.f90:28-142 for one force set (set 1)."""
         s, L = self.s, self.lists
         self.fx, self.fy, self.rm = self.forces()
         dxpos = np.zeros(self.np_)
         dypos = np.zeros(self.np_)
-        ampli = math.acos(-1.0) / 8.0
         alphav = self.disn2
-        self.disn2 = ampli * math.sin(10.0 * time)
+        self.disn2 = self.pitch_angle(time)
         alpha = self.disn2 - alphav
         yposr = 0.0
         iset = np.ascontiguousarray(L["iset"], np.int32)
@@ -360,6 +391,10 @@ class OracleSolver:
         s["U1"][:] = s["U"]
         self.rk(dtmin)
         self.fluid_structure(dtmin, self.time)
+        # FORCE_VISC as the print block calls it (ns2DComp.ALE.f90:231-236):
+        # after the mesh move, before the geometry update
+        if cs.fmu != 0.0:
+            self.fvx, self.fvy = self.force_visc()
         self.bandera += 1
         if cs.moving == 1:
             self.normales()
@@ -373,7 +408,8 @@ class OracleSolver:
         st = {k: v.copy() for k, v in self.s.items()}
         st.update(npoin=self.np_, nelem=self.ne, nnz=int(self.tab["lap_idx"].size),
                   DTMIN=self.dtmin, TIME=self.time, DTMIN1=self.dtmin1, BANDERA=float(self.bandera),
-                  ITER=float(self.iter), HMIN=self.hmin, FX1=self.fx, FY1=self.fy, RM1=self.rm)
+                  ITER=float(self.iter), HMIN=self.hmin, FX1=self.fx, FY1=self.fy, RM1=self.rm,
+                  FVX1=self.fvx, FVY1=self.fvy)
         return st
 
     def set_state(self, st: dict) -> None:

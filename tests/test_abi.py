@@ -24,9 +24,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_enums():
     L = _lib.lib()
-    assert L.cfd_abi_version() == 4
+    assert L.cfd_abi_version() == 5
     assert _lib.FIELDS["CFD_F_U"] == 0 and _lib.FIELDS["CFD_NFIELDS"] == 34
-    assert _lib.SCALARS["CFD_NSCALARS"] == 18
+    assert _lib.SCALARS["CFD_NSCALARS"] == 21
     assert C.sizeof(_lib.Consts) == 11 * 8
 
 

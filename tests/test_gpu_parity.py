@@ -50,6 +50,39 @@ def slip():
 
 
 @pytest.fixture(scope="module")
+def c1flow():
+    # the CPU-runnable configuration's flow: M 0.3, Re 100, FSAFE 0.5
+    cs = cylinder_case(64, 128, mach=0.3, re=100.0)
+    o = OracleSolver(cs)
+    for _ in range(40):
+        o.step()
+    return cs, o
+
+
+@pytest.fixture(scope="module")
+def itloc():
+    # local time stepping (ITLOCAL /= 0): DTL blends DTMIN with each element's DT
+    from cfd_amd.mesh import acoustic_fsafe
+    cs = cylinder_case(40, 80, mach=0.5, re=500.0, itlocal=25, fsafe=acoustic_fsafe(0.5))
+    o = OracleSolver(cs)
+    for _ in range(8):
+        o.step()
+    assert np.abs(o.s["DTL"] - o.dtmin).max() > 0          # the blend is active
+    return cs, o
+
+
+@pytest.fixture(scope="module")
+def pitch1():
+    # the parameterised pitching law alpha(t) = amp sin(omega t) (motion = 1)
+    cs = ellipse_case(32, 64, motion=1, motion_amp=0.2, motion_omega=60.0)
+    o = OracleSolver(cs)
+    for _ in range(10):
+        o.step()
+    assert o.disn2 != 0.0
+    return cs, o
+
+
+@pytest.fixture(scope="module")
 def mctx(cyl):
     from cfd_amd.api import MeshContext
     cs, _ = cyl
@@ -169,6 +202,50 @@ def test_fuente(cyl, mctx):
     s["W_X"][:], s["W_Y"][:] = save
     got = mctx.FUENTE(np.zeros((cs.npoin, 4)), s["U"], wx, wy, s["DNX"], s["DNY"], s["AREA"], s["DTL"])
     assert np.all(relnorm_cols(got, ref) < TOL)
+
+
+def test_force_visc_entry(cyl, mctx):
+    """cfd_force_visc (FORCE_VISC, ns2DComp.ALE.f90:819-893) against the
+    oracle on a developed state; the sums run in the same edge order."""
+    cs, o = cyl
+    s = o.s
+    iset = np.asarray(cs.lists["iset"], np.int32)
+    got = mctx.force_visc(iset, s["X"], s["Y"], s["P"], s["T"], s["VEL_X"], s["VEL_Y"], s["DNX"],
+                          s["DNY"], cs.t_inf)
+    ref = o.force_visc()
+    print("FORCE_VISC got", got, "ref", ref)
+    for a, b in zip(got, ref):
+        assert abs(a - b) <= 1e-13 * max(abs(b), 1e-300) or a == b
+    # an empty set gives zero, a bad element id is refused
+    assert mctx.force_visc(np.zeros((0, 3), np.int32), s["X"], s["Y"], s["P"], s["T"],
+                           s["VEL_X"], s["VEL_Y"], s["DNX"], s["DNY"], cs.t_inf) == (0.0, 0.0)
+    from cfd_amd._lib import CfdError
+    bad = iset[:2].copy()
+    bad[1, 2] = cs.nelem + 1
+    with pytest.raises(CfdError):
+        mctx.force_visc(bad, s["X"], s["Y"], s["P"], s["T"], s["VEL_X"], s["VEL_Y"], s["DNX"],
+                        s["DNY"], cs.t_inf)
+
+
+def test_normalvel_entry(slip):
+    """cfd_normalvel (NORMALVEL, subrutinas.f90:66-85) with the normals of
+    cfd_normales, against the oracle's, bit for bit; refused before
+    cfd_normales on a fresh mesh context."""
+    from cfd_amd._lib import CfdError
+    from cfd_amd.api import MeshContext
+    cs, o = slip
+    s = o.s
+    ctx = MeshContext(cs.mesh.inpoel, cs.mesh.npoin)
+    rng = np.random.default_rng(8)
+    vx, vy, wx, wy = (rng.standard_normal(cs.npoin) for _ in range(4))
+    with pytest.raises(CfdError):
+        ctx.normalvel(vx, vy, wx, wy)
+    ctx.normales(cs.lists, s["X"], s["Y"])
+    gx, gy = ctx.normalvel(vx, vy, wx, wy)
+    o.normales()
+    rx, ry = o.normalvel(vx, vy, wx, wy)
+    assert bitdiff(gx, rx) == 0 and bitdiff(gy, ry) == 0
+    assert bitdiff(gx, vx) > 0
 
 
 def test_rk(cyl, mctx):
@@ -349,7 +426,7 @@ def _session_from_oracle(cs, o, precision=0):
 
 
 @pytest.mark.parametrize("precision", [0, 2])
-@pytest.mark.parametrize("which", ["cyl", "ell", "slip"])
+@pytest.mark.parametrize("which", ["cyl", "ell", "slip", "c1flow", "itloc", "pitch1"])
 def test_session_teacher_forced(which, precision, request):
     """Each step from the oracle's state; the GPU step must land within 1e-12:
     precision 0 = the fused FP64 path (regrouped element arithmetic, patch
@@ -394,6 +471,17 @@ def assert_forces_match(cs, st, ref, got, tol=TOL):
     arm = np.abs(X[n1] - cs.xref) + np.abs(X[n2] - cs.xref) + np.abs(Y[n1] - cs.yref) + \
         np.abs(Y[n2] - cs.yref)
     scale = {"FX1": (pav * dy).sum(), "FY1": (pav * dx).sum(), "RM1": (pav * (dx + dy) * arm).sum()}
+    # FORCE_VISC: element-average pressure plus viscous stresses on each edge
+    if cs.fmu != 0.0:
+        el = iset[:, 2] - 1
+        vtx = cs.mesh.inpoel[el] - 1
+        pe = np.abs(P[vtx]).mean(axis=1)
+        g = sum(np.abs((ref["DNX"][el] if k == 0 else ref["DNY"][el]) * ref[v][vtx]).sum(axis=1)
+                for k in (0, 1) for v in ("VEL_X", "VEL_Y"))
+        ln = np.hypot(X[n1] - X[n2], Y[n1] - Y[n2])
+        vs = ((pe + 0.1 * g) * ln).sum()
+        scale.update(FVX1=vs, FVY1=vs)
+        assert abs(ref["FVX1"]) > 0
     for k, sc in scale.items():
         assert abs(got[k] - ref[k]) <= tol * sc, (k, got[k], ref[k], sc)
 

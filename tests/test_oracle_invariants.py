@@ -365,3 +365,119 @@ def test_deltat_single_element_closed_form():
     dtc = hh ** 2 / (4 * et)
     assert abs(d.value - 0.5 / (1 / dtc + 1 / dtu)) <= 1e-15 * d.value
     assert DT[0] == d.value
+
+
+def _sutherland(T, t_inf):
+    return 0.017 * (T / t_inf) ** 1.5 * (t_inf + 110.0) / (T + 110.0)
+
+
+def _visc_setup(nedges=9):
+    """Cylinder body edges 0..nedges-1 (an open chain from theta_0 to
+    theta_k) with the oracle's start-up geometry."""
+    cs = cylinder_case(10, 40)
+    o = orc.OracleSolver(cs)
+    iset = np.ascontiguousarray(cs.lists["iset"][:nedges], np.int32)
+    return cs, o, iset
+
+
+def _force_visc(o, iset, P, T, VX, VY, t_inf=288.15):
+    s = o.s
+    fvx, fvy = C.c_double(), C.c_double()
+    L().orc_force_visc(iset.shape[0], iset, o.inpoel, s["X"], s["Y"], P, T, VX, VY, s["DNX"],
+                       s["DNY"], t_inf, C.byref(fvx), C.byref(fvy))
+    return fvx.value, fvy.value
+
+
+def test_force_visc_linear_velocity_closed_form():
+    """FORCE_VISC (ns2DComp.ALE.f90:819-893) on a linear velocity field
+    u = a x + b y + c, v = d x + e y + f with uniform p and T: the P1
+    gradients are exact, so the stress tensor is constant and the sum over an
+    open chain of edges is sigma . (Y_0 - Y_k, X_k - X_0), the chain's chord
+    turned by the edge normal's convention (Y2-Y1, -(X2-X1))."""
+    cs, o, iset = _visc_setup()
+    X, Y = o.s["X"], o.s["Y"]
+    a, b, c, d, e, f = 0.7, -1.3, 2.0, 0.4, 0.9, -0.5
+    VX, VY = a * X + b * Y + c, d * X + e * Y + f
+    p, T = 3.5, 250.0
+    fvx, fvy = _force_visc(o, iset, np.full(cs.npoin, p), np.full(cs.npoin, T), VX, VY)
+    mu = _sutherland(T, 288.15)
+    div = a + e
+    sxx = -p - mu * (2.0 / 3.0 * div - 2.0 * a)
+    syy = -p - mu * (2.0 / 3.0 * div - 2.0 * e)
+    sxy = mu * (b + d)
+    # edge q runs n1 -> n2 with n2(q+1) = n1(q): the sum of (Y2-Y1, -(X2-X1))
+    # telescopes to the difference between n2 of the first edge and n1 of the last
+    a0, b0 = iset[0, 1] - 1, iset[-1, 0] - 1
+    assert np.array_equal(iset[1:, 1], iset[:-1, 0])
+    sx, sy = Y[a0] - Y[b0], -(X[a0] - X[b0])
+    ex, ey = sxx * sx + sxy * sy, sxy * sx + syy * sy
+    scale = (abs(sxx) + abs(sxy) + abs(syy)) * (abs(sx) + abs(sy))
+    assert abs(fvx - ex) <= 1e-13 * scale and abs(fvy - ey) <= 1e-13 * scale
+    # the viscous part alone (p = 0) is non-trivial, so a dropped viscous term fails
+    fv0 = _force_visc(o, iset, np.zeros(cs.npoin), np.full(cs.npoin, T), VX, VY)
+    vsc = (abs(sxx + p) + abs(sxy) + abs(syy + p)) * (abs(sx) + abs(sy))
+    assert abs(fv0[0] - (ex + p * sx)) <= 1e-13 * vsc and abs(fv0[0]) > 0.1 * vsc
+    assert abs(fv0[1] - (ey + p * sy)) <= 1e-13 * vsc
+
+
+def test_force_visc_element_average_pressure_and_edge_temperature():
+    """Zero velocity and a linear pressure: each edge carries the pressure at
+    its element's centroid (the element average), not at the edge midpoint.
+    With a linear temperature and a uniform shear the viscosity follows the
+    edge-midpoint temperature (the edge average)."""
+    cs, o, iset = _visc_setup(15)
+    X, Y = o.s["X"], o.s["Y"]
+    z = np.zeros(cs.npoin)
+    P = 2.0 + 0.3 * X - 0.7 * Y
+    fvx, fvy = _force_visc(o, iset, P, np.full(cs.npoin, 288.15), z, z)
+    n1, n2, el = iset[:, 0] - 1, iset[:, 1] - 1, iset[:, 2] - 1
+    vtx = o.inpoel[el] - 1
+    xc, yc = X[vtx].mean(axis=1), Y[vtx].mean(axis=1)
+    pc = 2.0 + 0.3 * xc - 0.7 * yc
+    lx, ly = Y[n2] - Y[n1], -(X[n2] - X[n1])          # (RLX, RLY) times RMOD
+    ex, ey = -(pc * lx).sum(), -(pc * ly).sum()
+    sc = (np.abs(pc) * (np.abs(lx) + np.abs(ly))).sum()
+    assert abs(fvx - ex) <= 1e-13 * sc and abs(fvy - ey) <= 1e-13 * sc
+    # the edge-midpoint pressure gives a different answer on this chain
+    pm = 2.0 + 0.3 * (X[n1] + X[n2]) / 2 - 0.7 * (Y[n1] + Y[n2]) / 2
+    assert abs(-(pm * lx).sum() - ex) > 1e-6 * sc
+    # uniform shear u = y, temperature linear in x: mu at each edge midpoint
+    T = 200.0 + 30.0 * X
+    fvx, fvy = _force_visc(o, iset, z, T, Y.copy(), z)
+    mu = _sutherland((T[n1] + T[n2]) / 2.0, 288.15)
+    ex, ey = (mu * ly).sum(), (mu * lx).sum()           # sigma_xy = mu, sigma_xx = sigma_yy = 0
+    sc = (mu * (np.abs(lx) + np.abs(ly))).sum()
+    assert abs(fvx - ex) <= 1e-13 * sc and abs(fvy - ey) <= 1e-13 * sc
+
+
+def test_force_visc_closed_body_uniform_state():
+    """Uniform pressure, temperature and velocity on the closed body: zero
+    velocity gradients and a closed chain of normals give F_V = 0."""
+    cs = cylinder_case(10, 40)
+    o = orc.OracleSolver(cs)
+    iset = np.ascontiguousarray(cs.lists["iset"], np.int32)
+    n = cs.npoin
+    fvx, fvy = _force_visc(o, iset, np.full(n, 5.0), np.full(n, 300.0), np.full(n, 3.0),
+                           np.full(n, -1.0))
+    assert abs(fvx) < 1e-12 and abs(fvy) < 1e-12
+
+
+def test_normalvel_projection():
+    """NORMALVEL (subrutinas.f90:66-85): on the slip-wall nodes the velocity
+    relative to the mesh loses its normal component (v - w) . n = 0, keeps
+    its tangential component, and other nodes are untouched."""
+    cs = cylinder_case(10, 40, slip_body=True)
+    o = orc.OracleSolver(cs)
+    n_ip, nx, ny = o.norm
+    assert n_ip.size == cs.mesh.nt
+    rng = np.random.default_rng(3)
+    vx, vy, wx, wy = (rng.standard_normal(cs.npoin) for _ in range(4))
+    px, py = o.normalvel(vx, vy, wx, wy)
+    k = n_ip - 1
+    rn = (px[k] - wx[k]) * nx + (py[k] - wy[k]) * ny
+    assert np.abs(rn).max() < 1e-14
+    t_before = -(vx[k] - wx[k]) * ny + (vy[k] - wy[k]) * nx
+    t_after = -(px[k] - wx[k]) * ny + (py[k] - wy[k]) * nx
+    assert np.allclose(t_after, t_before, rtol=1e-14, atol=1e-14)
+    other = np.setdiff1d(np.arange(cs.npoin), k)
+    assert np.array_equal(px[other], vx[other]) and np.array_equal(py[other], vy[other])
