@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libcfd_amd.so")
 SOURCES = ["tables.hip", "geom.hip", "flow.hip", "pcg.hip", "patches.hip", "mlp.hip",
-           "warmstart.hip", "state.hip", "batch.hip", "api.hip"]
+           "warmstart.hip", "state.hip", "batch.hip", "api.hip", "restart.hip"]
 HEADERS = ["cfd_internal.hpp", "reduce.hpp", "crmath.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -99,7 +99,7 @@ def build_reference() -> bool:
     and the Fortran shims against the reference's own module sources."""
     if not os.path.isdir("/root/reference") or shutil.which("flang") is None:
         return False
-    _run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile.ref", "all", "shims"])
+    _run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile.ref", "all", "shims", "shimharness"])
     return True
 
 

@@ -13,6 +13,13 @@
 !   ref_harness init    out.bin          state after start-up
 !   ref_harness run  N  in.bin out.bin   N steps from a state (or "-" = start-up),
 !                                        appending the state after every step
+!   ref_harness rstwrite in.bin ITER     PRINTREST (ns2DComp.ALE.f90:898-917) of a
+!                                        state: writes <name>.RST
+!   ref_harness rstread out.bin          RESTART with IRESTART = 1 (:394-435):
+!                                        U, T, GAMM from <name>.RST, then the state
+!   ref_harness time N                   N steps from the start-up; prints their
+!                                        wall time and the reference's own calcRHS
+!                                        timer (calcrhs_t) over them
 !   ref_harness <unit>  in.bin out.bin   one routine on a state:
 !       deltat estab calcrhs fuente rk deriv masas laplace bicg fluid gcl normales
 program ref_harness
@@ -63,6 +70,17 @@ program ref_harness
    case ('tables')
       call dump_tables(trim(arg2))
    case ('init')
+      call save_state(trim(arg2), .false.)
+   case ('time')
+      read(arg2, *) nsteps
+      call time_steps(nsteps)
+   case ('rstwrite')
+      call load_state(trim(arg2))
+      read(arg3, *) i
+      call PRINTREST(i, npoin, U, T, GAMM, TIME, FILENAME)
+   case ('rstread')
+      IRESTART = 1
+      call RESTART(GAMM)
       call save_state(trim(arg2), .false.)
    case ('run')
       read(arg2, *) nsteps
@@ -159,6 +177,22 @@ contains
          U(:, ipoin) = U1(:, ipoin)
       end do
    end subroutine one_step
+
+   subroutine time_steps(n)
+      ! n steps timed with system_clock, as the reference's timer macro does
+      ! (the start-up has already written every array once)
+      integer, intent(in) :: n
+      integer(8) :: c0, c1, rate
+      real(8) :: t_calc0
+      t_calc0 = calcrhs_t
+      call system_clock(c0, rate)
+      do istep = 1, n
+         call one_step()
+      end do
+      call system_clock(c1)
+      write(*, '(A, I8, A, ES16.8, A, ES16.8)') 'timed_steps ', n, ' wall_s ', &
+         dble(c1 - c0)/dble(rate), ' calcrhs_s ', dble(calcrhs_t - t_calc0)
+   end subroutine time_steps
 
    subroutine unit_call(name)
       character(len=*), intent(in) :: name

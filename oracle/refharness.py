@@ -16,6 +16,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_BIN = os.path.join(HERE, "_ref", "ref_harness")
+# the same harness linked against the Fortran shims (libcfd_amd.so) instead of
+# the reference's hot-path objects (oracle/Makefile.ref shimharness)
+SHIM_BIN = os.path.join(HERE, "_ref", "shim_harness")
 
 SCALARS = ["DTMIN", "TIME", "DTMIN1", "BANDERA", "ITER", "HMIN", "FX1", "FY1", "RM1", "_pad"]
 # (name, kind, width): kind n = npoin, e = nelem, z = nnz
@@ -113,9 +116,10 @@ def read_tables(path: str) -> dict:
 class RefCase:
     """A case directory the reference harness can run in."""
 
-    def __init__(self, case, workdir: str | None = None):
+    def __init__(self, case, workdir: str | None = None, binary: str = REF_BIN):
         from cfd_amd.mesh import write_reference_inputs
         self.case = case
+        self.binary = binary
         self._tmp = None
         if workdir is None:
             self._tmp = tempfile.TemporaryDirectory(prefix="cfdref_")
@@ -125,10 +129,11 @@ class RefCase:
 
     def _run(self, *args, threads=1):
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        r = subprocess.run([REF_BIN, *map(str, args)], cwd=self.dir, env=env,
-                           capture_output=True, text=True)
+        r = subprocess.run([self.binary, *map(str, args)], cwd=self.dir, env=env,
+                           capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
-            raise RuntimeError(f"ref_harness {args} failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+            raise RuntimeError(f"{os.path.basename(self.binary)} {args} failed: "
+                               f"{r.stdout[-2000:]} {r.stderr[-2000:]}")
         return r
 
     def path(self, name):

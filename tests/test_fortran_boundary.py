@@ -28,8 +28,8 @@ REPLACED = {
     "mLaplace.o": ["_QMmlaplacePlaplace"],
     "calcRHS.o": ["_QMcalcrhs_modPcalcrhs"],
     "gcl.o": ["_QMgcl_modPmain", "_QMgcl_modPputw", "_QMgcl_modPputarea"],
-    "subrutinas.o": ["_QMmnormalesPnormales", "deriv_", "masas_", "deltat_", "estab_", "fuente_",
-                     "rk_"],
+    "subrutinas.o": ["_QMmnormalesPnormales", "_QMmnormalesPnormalvel", "deriv_", "masas_",
+                     "deltat_", "estab_", "fuente_", "rk_"],
 }
 
 
@@ -176,3 +176,111 @@ def test_fortran_session_driver(tmp_path, precision):
     for k in ("DTMIN", "TIME", "ITER", "FX1", "FY1"):
         assert sc[SCALARS["CFD_S_" + {"FX1": "FX", "FY1": "FY"}.get(k, k)]] == ref[k], k
     S.close()
+
+
+@pytest.mark.ref
+def test_kept_subrutinas_compile_against_the_shims(tmp_path):
+    """The routines of subrutinas.f90 that a drop-in keeps (CUARTO_ORDEN,
+    fixvel, FIX, ADAMSB: everything the shims do not replace) compile
+    against the shim modules: ADAMSB's `use Mnormales` and its NORMALVEL call
+    resolve to the shim's Mnormales.  The kept lines are cut from the
+    reference source into a temporary directory at test time (nothing of the
+    reference is stored in this repository) and linked with the shim object
+    into a relocatable object that defines adamsb_."""
+    if not os.path.isdir("/root/reference") or shutil.which("flang") is None:
+        pytest.skip("needs the reference sources and flang")
+    subprocess.run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile.ref", "shims"], check=True,
+                   capture_output=True)
+    lines = open("/root/reference/subrutinas.f90").read().splitlines(keepends=True)
+
+    import re
+
+    def block(name):
+        start = re.compile(r"\s*subroutine\s+" + name + r"\b", re.I)
+        end = re.compile(r"\s*end\s+subroutine\s+" + name + r"\b", re.I)
+        lo = next(i for i, l in enumerate(lines) if start.match(l))
+        hi = next(i for i in range(lo, len(lines)) if end.match(lines[i]))
+        return "".join(lines[lo:hi + 1])
+
+    kept = lines[0] + "".join(block(n) for n in ("CUARTO_ORDEN", "fixvel", "FIX", "ADAMSB"))
+    assert "call normalvel" in kept.lower() and "use mnormales" in kept.lower()
+    src = tmp_path / "kept.f90"
+    src.write_text(kept)
+    shims = os.path.join(ROOT, "oracle", "_ref", "shims")
+    r = subprocess.run(["flang", "-cpp", "-O0", "-I", shims, "-I", os.path.join(shims, "ref"), "-c",
+                        str(src), "-o", str(tmp_path / "kept.o")], capture_output=True, text=True,
+                       cwd=tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    undefined = subprocess.run(["nm", "--undefined-only", str(tmp_path / "kept.o")],
+                               capture_output=True, text=True, check=True).stdout
+    assert "_QMmnormalesPnormalvel" in undefined            # resolved by the shim object
+    assert "_QMmnormalesPnormalvel" in _defined(SHIM_OBJ)
+    assert "adamsb_" in _defined(str(tmp_path / "kept.o"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["slip", "ell"])
+def test_reference_calling_sequence_through_the_shims(which, tmp_path):
+    """The drop-in run: the reference's loader, start-up and step sequence
+    (the harness program, with fluidStructure compiled from meshMove.f90)
+    linked against the shim modules, so every hot-path call (NORMALES, DERIV,
+    MASAS, laplace, DELTAT, ESTAB, calcRHS, FUENTE, RK, biCG, NORMALVEL, GCL)
+    goes through cfd_shims.f90 and the C ABI to the GPU.  Against the
+    oracle: start-up geometry bit for bit; each unit on the oracle's
+    developed state within 1e-12; three whole steps from the start-up within
+    1e-12 of the oracle's (whole runs, as the harness keeps the previous
+    pitch angle inside MeshMove, outside the state file)."""
+    from cfd_amd.mesh import acoustic_fsafe, cylinder_case, ellipse_case
+    from oracle import refharness as rh
+    from oracle.oracle import OracleSolver
+    from tests.helpers import bitdiff, relnorm, relnorm_cols
+    if not os.path.isfile(rh.SHIM_BIN):
+        pytest.skip("shim harness not built (needs the reference sources and flang at build time)")
+    cs = (cylinder_case(32, 64, mach=0.5, re=500.0, slip_body=True, fsafe=acoustic_fsafe(0.5))
+          if which == "slip" else ellipse_case(24, 48))
+    rc = rh.RefCase(cs, str(tmp_path), binary=rh.SHIM_BIN)
+    o = OracleSolver(cs)
+    init = rc.init_state()
+    print(f"shim harness start-up on {cs.nelem} triangles done", flush=True)
+    for k in ("AREA", "DNX", "DNY", "HH", "M", "LAP_SPARSE", "LAP_DIAG"):
+        assert bitdiff(init[k], o.s[k]) == 0, k
+    for _ in range(6):
+        o.step()
+    st = o.state()
+    # single units on the oracle's developed state
+    for unit, fields in (("deltat", ("DT",)), ("estab", ("SHOC", "T_SUGN1", "T_SUGN2", "T_SUGN3")),
+                         ("rk", ("U1", "RHS", "T", "VEL_X", "VEL_Y", "P")),
+                         ("normales", ("VEL_X", "VEL_Y"))):
+        got = rc.unit(unit, st)
+        ref_o = OracleSolver(cs)
+        ref_o.set_state(st)
+        ref_o.disn2 = o.disn2
+        if unit == "deltat":
+            ref_o.deltat()
+        elif unit == "estab":
+            ref_o.estab(st["DTMIN"])
+        elif unit == "rk":
+            ref_o.rk(st["DTMIN"])
+        else:
+            ref_o.normales()
+            vx, vy = ref_o.normalvel(st["VEL_X"], st["VEL_Y"], st["W_X"], st["W_Y"])
+            ref_o.s["VEL_X"][:], ref_o.s["VEL_Y"][:] = vx, vy
+        ref = ref_o.state()
+        for f in fields:
+            err = relnorm_cols(got[f], ref[f]) if np.ndim(ref[f]) == 2 else [relnorm(got[f], ref[f])]
+            assert np.all(np.asarray(err) < 1e-12), (unit, f, err)
+        print(f"shim unit {unit}: ok", flush=True)
+    # three whole steps of the reference's sequence from the start-up
+    runs = rc.run(3)
+    o2 = OracleSolver(cs)
+    for k, got in enumerate(runs):
+        o2.step()
+        ref = o2.state()
+        assert got["DTMIN"] == ref["DTMIN"]
+        for f in ("U", "RHS"):
+            assert np.all(relnorm_cols(got[f], ref[f]) < 1e-12), (k, f)
+        for f in ("T", "P", "X", "Y", "M"):
+            assert relnorm(got[f], ref[f]) < 1e-12, (k, f)
+        if which == "ell":
+            assert relnorm(ref["X"], cs.mesh.x) > 0           # the mesh moved
+    print(f"shim harness: three steps match the oracle ({which})", flush=True)
