@@ -66,30 +66,44 @@ void set_last_error(const std::string& msg);
 
 #define CFD_ARG(cond, msg) CFD_CHECK(cond, std::string("arg: ") + msg)
 
-// Owned device allocation.
+// Owned device allocation, or (borrow) a view of memory owned elsewhere,
+// which release() leaves alone.
 template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), owned(o.owned) { o.p = nullptr; o.n = 0; }
     DBuf& operator=(DBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; owned = o.owned;
+            o.p = nullptr; o.n = 0;
+        }
         return *this;
     }
     ~DBuf() { release(); }
     void alloc(size_t count) {
         release();
         n = count;
+        owned = true;
         if (count) CFD_HIP(hipMalloc(&p, sizeof(T) * count));
     }
+    void borrow(T* ptr, size_t count) {
+        release();
+        p = ptr;
+        n = count;
+        owned = false;
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = true;
     }
     void upload(const T* h, size_t count, hipStream_t s) {
         CFD_CHECK(count <= n, "upload larger than buffer");
@@ -497,7 +511,7 @@ struct PcgWork {
     DBuf<double> r, z, y, pa, pb, partial;
     DBuf<double> scal;         // see pcg.hip for the slot meaning
     DBuf<int> fixflag;         // n: 1 on Dirichlet nodes
-    HostPinned<double> hpin;   // host copy of scal for the polls
+    HostPinned<double> hpin;   // host copy of scal for the polls (null: a stack copy)
     void ensure(int npoin);
 };
 // One system of a multi-system solve: its own work space, x, b and fixed

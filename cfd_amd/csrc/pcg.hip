@@ -36,6 +36,8 @@
 // the residual of the guess that is zero off the fixed nodes, so the stopping
 // point does not depend on the initial guess (warm starts compare at equal
 // tolerance).
+#include <mutex>
+
 #include "cfd_internal.hpp"
 #include "reduce.hpp"
 
@@ -441,8 +443,10 @@ void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A
                       const int* rowptr, const double* diag, double tol, int maxit, int rel,
                       hipStream_t s) {
     CFD_CHECK(npoin > 0, "arg: PCG needs npoin > 0");
+    constexpr int kMaxSys = 2;
+    CFD_CHECK(nsys >= 1 && nsys <= kMaxSys, "internal: PCG systems per launch");
     const dim3 g(grid_for(npoin)), blk(kBlock);
-    std::vector<int> cur(nsys, 0);  // pbuf[cur] holds the current p
+    int cur[kMaxSys] = {0, 0};  // pbuf[cur] holds the current p
     const int nb = (int)g.x;
     const dim3 one(1), fin(256);
     auto iter_a = [&](int k) {
@@ -476,16 +480,20 @@ void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A
         cur[k] = 0;
     }
     CFD_HIP(hipGetLastError());
-    std::vector<char> done(nsys, 0), stop(nsys, 0);
+    bool done[kMaxSys] = {false, false}, stop[kMaxSys] = {false, false};
+    double local[kMaxSys][S_NSCAL];  // the polls' host copy where no pinned buffer exists
     auto poll = [&]() {
-        for (int k = 0; k < nsys; ++k)
-            CFD_HIP(hipMemcpyAsync(sys[k].w->hpin.p, sys[k].w->scal.p, S_NSCAL * sizeof(double),
+        for (int k = 0; k < nsys; ++k) {
+            double* h = sys[k].w->hpin.p ? sys[k].w->hpin.p : local[k];
+            CFD_HIP(hipMemcpyAsync(h, sys[k].w->scal.p, S_NSCAL * sizeof(double),
                                    hipMemcpyDeviceToHost, s));
+        }
         CFD_HIP(hipStreamSynchronize(s));
         bool all = true;
         for (int k = 0; k < nsys; ++k) {
-            done[k] = sys[k].w->hpin.p[S_DONE] != 0.0;
-            stop[k] = sys[k].w->hpin.p[S_STOP] != 0.0;
+            const double* h = sys[k].w->hpin.p ? sys[k].w->hpin.p : local[k];
+            done[k] = h[S_DONE] != 0.0;
+            stop[k] = h[S_STOP] != 0.0;
             all = all && done[k];
         }
         return all;
@@ -554,3 +562,122 @@ PcgResult pcg_result(const PcgWork& w, hipStream_t s) {
 }
 
 }  // namespace cfd
+
+// ---------------------------------------------------------------------------
+// The same solve with no allocation (cfd_bicg_ex_noalloc): every work vector
+// lives in one file-scope device array, `scratch`, sized for kNoAllocRows
+// rows; its address on each device comes from hipGetSymbolAddress once, with
+// that device current, and only that address is handed to kernels and copies.
+// Calls on one device share the array: overlapping calls there race.
+// ---------------------------------------------------------------------------
+namespace cfd {
+namespace {
+
+constexpr int kNoAllocRows = 1 << 19;  // 524,288 rows
+constexpr int kNoAllocBlocks = (kNoAllocRows + kBlock - 1) / kBlock;
+constexpr size_t kAlign = 32;  // doubles: 256-byte sub-array alignment
+constexpr size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+// layout (in doubles): r, z, y, pa, pb (kNoAllocRows each), partial
+// (3 kNoAllocBlocks), scal (S_NSCAL), fixflag (kNoAllocRows int32), the CSR
+// check flag (one int32, in its own slot)
+constexpr size_t kOffVec = 0;
+constexpr size_t kOffPartial = align_up(kOffVec + 5 * (size_t)kNoAllocRows);
+constexpr size_t kOffScal = align_up(kOffPartial + 3 * (size_t)kNoAllocBlocks);
+constexpr size_t kOffFix = align_up(kOffScal + S_NSCAL);
+constexpr size_t kOffFlag = align_up(kOffFix + kNoAllocRows / 2);
+constexpr size_t kScratchDoubles = align_up(kOffFlag + 1);
+
+}  // namespace
+
+// about 23 MB per device, held until the process ends
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+double* scratch_on_current_device() {
+    int dev = 0;
+    CFD_HIP(hipGetDevice(&dev));
+    CFD_CHECK(dev >= 0 && dev < kMaxDevices, "device id beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        CFD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        CFD_CHECK(ptr != nullptr, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// The row pointer starts at 0 and never decreases, every column index and
+// fixed-node index lies in 1..n: else *flag = 1 (flag cleared by the caller's
+// first launch, block 0 thread 0 of k_csr_flag_clear).
+__global__ void k_csr_flag_clear(int* __restrict__ flag) { *flag = 0; }
+
+__global__ void k_csr_check(int n, const int* __restrict__ rowptr, const int* __restrict__ idx,
+                            const int* __restrict__ fix_idx, int nfix, int* __restrict__ flag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    if (t == 0 && rowptr[0] != 0) *flag = 1;
+    for (int i = t; i < n; i += nt)
+        if (rowptr[i + 1] < rowptr[i]) *flag = 1;
+    const int nnz = rowptr[n];
+    for (int q = t; q < nnz; q += nt) {
+        const int c = idx[q];
+        if (c < 1 || c > n) *flag = 1;
+    }
+    for (int q = t; q < nfix; q += nt) {
+        const int c = fix_idx[q];
+        if (c < 1 || c > n) *flag = 1;
+    }
+}
+
+}  // namespace
+}  // namespace cfd
+
+extern "C" int cfd_bicg_ex_noalloc(int npoin, const double* spMtx, const int* spIdx,
+                                   const int* spRowptr, const double* diagMtx, double* x,
+                                   const double* b, const double* x_fix, const int* x_fixIdx,
+                                   int nfix, double tol, int maxit, int rel, void* stream,
+                                   int* iters, double* rz, int* converged) {
+    using namespace cfd;
+    CFD_TRY({
+        CFD_ARG(npoin > 0 && spMtx && spIdx && spRowptr && diagMtx && x && b, "null argument");
+        CFD_ARG(npoin <= kNoAllocRows, "npoin " + std::to_string(npoin) + " exceeds the " +
+                                           std::to_string(kNoAllocRows) +
+                                           " rows of the allocation-free scratch");
+        CFD_ARG(tol > 0.0 && maxit >= 0 && (rel == 0 || rel == 1), "tol > 0, maxit >= 0, rel 0/1");
+        CFD_ARG(nfix >= 0 && (nfix == 0 || (x_fix && x_fixIdx)), "fixed-node lists");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        double* base = scratch_on_current_device();
+        // the CSR and fixed-node indices are checked on the device first
+        int* flag = reinterpret_cast<int*>(base + kOffFlag);
+        k_csr_flag_clear<<<1, 1, 0, s>>>(flag);
+        k_csr_check<<<1024, kBlock, 0, s>>>(npoin, spRowptr, spIdx, x_fixIdx, nfix, flag);
+        CFD_HIP(hipGetLastError());
+        int bad = 0;
+        CFD_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        CFD_HIP(hipStreamSynchronize(s));
+        CFD_ARG(bad == 0, "row pointer, column index or fixed-node index out of range");
+        PcgWork w;  // views of the scratch array; nothing is allocated
+        const int nb = (int)grid_for(npoin);
+        w.n = npoin;
+        w.nblocks = nb;
+        w.r.borrow(base + kOffVec, npoin);
+        w.z.borrow(base + kOffVec + (size_t)kNoAllocRows, npoin);
+        w.y.borrow(base + kOffVec + 2 * (size_t)kNoAllocRows, npoin);
+        w.pa.borrow(base + kOffVec + 3 * (size_t)kNoAllocRows, npoin);
+        w.pb.borrow(base + kOffVec + 4 * (size_t)kNoAllocRows, npoin);
+        w.partial.borrow(base + kOffPartial, 3 * (size_t)nb);
+        w.scal.borrow(base + kOffScal, S_NSCAL);
+        w.fixflag.borrow(reinterpret_cast<int*>(base + kOffFix), npoin);
+        pcg_launch(w, npoin, spMtx, spIdx, spRowptr, diagMtx, x, b, x_fix, x_fixIdx, nfix, tol,
+                   maxit, rel, s);
+        PcgResult res = pcg_result(w, s);
+        if (iters) *iters = res.iters;
+        if (rz) *rz = res.rz;
+        if (converged) *converged = res.converged;
+    })
+}

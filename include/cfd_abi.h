@@ -164,6 +164,20 @@ int cfd_bicg_ex(int npoin, const double *spMtx, const int *spIdx, const int *spR
                 const int *x_fixIdx, int nfix, double tol, int maxit, int rel, int *iters,
                 double *rz, int *converged);
 
+/* cfd_bicg_ex with no allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops): no counterpart in
+ * the reference.  Every array is a DEVICE pointer (the caller owns them);
+ * the solve's work vectors live in a static device array of the library
+ * sized for npoin <= 524,288 rows (larger systems are refused with
+ * CFD_ERR_ARG before any work); stream is a hipStream_t (null: the default
+ * stream); the call returns when the solve has ended.  The row pointer,
+ * column and fixed-node indices are checked on the device first.  Calls on
+ * one GPU share its scratch array: overlapping calls there race. */
+int cfd_bicg_ex_noalloc(int npoin, const double *spMtx, const int *spIdx, const int *spRowptr,
+                        const double *diagMtx, double *x, const double *b, const double *x_fix,
+                        const int *x_fixIdx, int nfix, double tol, int maxit, int rel,
+                        void *stream, int *iters, double *rz, int *converged);
+
 /* The mesh-displacement MLP of the learned warm start (no counterpart in the
  * reference): 16 -> 64 -> 64 -> 16 on the matrix cores, BF16 operands, FP32
  * accumulation, tanh hidden layers; params in the layout W1t[64][16], b1[64],
