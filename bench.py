@@ -302,7 +302,11 @@ def batch_sweep(steps: int = 100, warmup: int = 5, ncases: int = 256) -> dict:
             "states_finite": finite}
 
 
-def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 100000) -> dict:
+LARGE_RUNS = ("plain_fp64", "extrap_fp32", "learned_fp32")
+
+
+def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 100000,
+              runs_wanted=LARGE_RUNS) -> dict:
     """The large ALE run: pitching ellipse O-grid 4096 x 4096 (16,777,216 nodes,
     33,546,240 triangles), M=0.7, Re=10000, alpha(t) = 5 deg sin(0.2 t), the
     GCL volume update on; plain (fused FP64, previous-step guess) against
@@ -319,6 +323,8 @@ def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 
                       motion_omega=PITCH_OMEGA)
     runs = {}
     for name, prec, ws in (("plain_fp64", 0, 0), ("extrap_fp32", 1, 2), ("learned_fp32", 1, 1)):
+        if name not in runs_wanted:
+            continue
         t0 = time.perf_counter()
         S = Session(cs, precision=prec, pcg_rel=1, pcg_tol=tol, pcg_maxit=maxit, warm_start=ws,
                     gcl=1)
@@ -344,12 +350,13 @@ def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 
         if ws:
             runs[name]["predicted_guess_used"] = f"{sc['WS_LEARNED']:.0f}/{sc['WS_SOLVES']:.0f}"
         S.close()
-    p = runs["plain_fp64"]
-    runs["speedup"] = p["ms_per_step"] / runs["learned_fp32"]["ms_per_step"]
-    runs["speedup_extrap"] = p["ms_per_step"] / runs["extrap_fp32"]["ms_per_step"]
-    for k in ("extrap_fp32", "learned_fp32"):
-        runs[f"{k}_cd_diff"] = _rel(runs[k]["cd"], p["cd"])
-        runs[f"{k}_cl_diff"] = _rel(runs[k]["cl"], p["cl"])
+    if all(k in runs for k in LARGE_RUNS):
+        p = runs["plain_fp64"]
+        runs["speedup"] = p["ms_per_step"] / runs["learned_fp32"]["ms_per_step"]
+        runs["speedup_extrap"] = p["ms_per_step"] / runs["extrap_fp32"]["ms_per_step"]
+        for k in ("extrap_fp32", "learned_fp32"):
+            runs[f"{k}_cd_diff"] = _rel(runs[k]["cd"], p["cd"])
+            runs[f"{k}_cl_diff"] = _rel(runs[k]["cl"], p["cl"])
     runs["workload"] = (f"ellipse O-grid 4096x4096 ({cs.npoin} nodes, {cs.nelem} triangles), M=0.7, "
                         f"Re=10000, alpha(t) = 5 deg sin(0.2 t), GCL on, relative PCG rule {tol:g} "
                         f"(at most {maxit} iterations), {warmup} untimed + {steps} timed steps")
@@ -495,6 +502,12 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--no-large-ale", action="store_true",
                     help="skip the 4096 x 4096 pitching ellipse (about two minutes)")
+    ap.add_argument("--large-ale-runs", default=",".join(LARGE_RUNS),
+                    help="comma-separated runs of the large ALE section (profiling)")
+    ap.add_argument("--large-ale-steps", type=int, default=2)
+    ap.add_argument("--large-ale-warmup", type=int, default=3)
+    ap.add_argument("--only", choices=["large_ale", "ale"], default=None,
+                    help="run only this section and print its JSON (profiling)")
     ap.add_argument("--develop", type=int, default=500,
                     help="untimed FP64 steps from the free stream before timing (the state timed)")
     ap.add_argument("--no-renumber", action="store_true",
@@ -505,6 +518,13 @@ def main():
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "large_ale":
+        print(json.dumps({"large_ale": large_ale(args.large_ale_steps, args.large_ale_warmup,
+                                                 runs_wanted=args.large_ale_runs.split(","))}))
+        return
+    if args.only == "ale":
+        print(json.dumps({"ale": ale_compare(args.ale_steps)}))
+        return
     cs = config2_case(args.nr, args.nt)
     prec = PRECISION[args.precision]
     rn = not args.no_renumber
@@ -574,7 +594,8 @@ def main():
         out["batch"] = batch_sweep(args.batch_steps)
         progress("batch sweep done")
     if not args.no_large_ale:
-        out["large_ale"] = large_ale()
+        out["large_ale"] = large_ale(args.large_ale_steps, args.large_ale_warmup,
+                                     runs_wanted=args.large_ale_runs.split(","))
         progress("large ALE done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cs)

@@ -624,6 +624,8 @@ struct cfd_session {
     DBuf<int> ILAUX, ISET;
     BcDevice bc;
     PcgWork pcg, pcg_y;  // the x and y mesh solves
+    EllLap ell;          // the Laplace values again in ELL order, for the solves' SpMV
+    bool ell_on = false;
     WarmStart ws;        // learned warm start (warm_start = 1, moving nodes)
     bool ws_on = false;
     DBuf<double> ctl;                  // session scalars on the device (X_NCTL slots)
@@ -708,6 +710,7 @@ struct cfd_session {
         refresh_patch_geometry();
         launch_masas(t, AREA.p, M.p, s);
         launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
+        if (ell_on) ell.refresh(t, LAP.p, s);
     }
 
     // the fused paths' ESTAB outputs (patch order)
@@ -781,8 +784,9 @@ struct cfd_session {
         }
         const PcgSystem sys[2] = {{&pcg, XPOS.p, BZERO.p, POSAUX.p, ILAUX.p, nn},
                                   {&pcg_y, YPOS.p, BZERO.p, POSAUY.p, ILAUX.p, nn}};
+        const EllMat em = ell.mat(m->t);
         pcg_launch_multi(sys, 2, np, LAP.p, m->t.idx.p, m->t.rowptr.p, LDIAG.p, tol, maxit,
-                         d.pcg_rel, s);
+                         d.pcg_rel, s, ell_on ? &em : nullptr);
         last_solves_trivial = pcg_polled_iterations(pcg) == 0.0 && pcg_polled_iterations(pcg_y) == 0.0;
         te(CFD_K_PCG);
         pcg_store_result(pcg, scal(CFD_S_PCG_ITX), scal(CFD_S_PCG_RZX), scal(CFD_S_PCG_ITSUM), s,
@@ -895,6 +899,7 @@ struct cfd_session {
                 launch_gcl(t, M.p, WX.p, WY.p, WXO.p, WYO.p, DNX.p, DNY.p, AREA.p, AREA_OLD.p, dtmin,
                            d.gcl, s);
                 launch_laplace(t, X.p, Y.p, DNX.p, DNY.p, LAP.p, LDIAG.p, s);
+                if (ell_on) ell.refresh(t, LAP.p, s);
             } else {
                 geometry();
             }
@@ -1149,6 +1154,8 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->bc.upload(h, np, &d->bc, s);
         S->pcg.ensure(np);
         S->pcg_y.ensure(np);
+        // the ELL copy serves the mesh solves, which iterate only with moving nodes
+        if (d->nmove > 0) S->ell_on = S->ell.build(m->t, s);
         S->ws_on = d->warm_start != 0 && d->nmove > 0;
         if (S->ws_on) {  // length scale: farthest node from the pivot
             double l = 0.0;
@@ -1285,6 +1292,10 @@ int cfd_session_set_field(cfd_session* S, int f, const double* host_in) {
             S->move_normalized = false;
         if (f == CFD_F_DNX || f == CFD_F_DNY) {
             S->refresh_patch_geometry();
+            CFD_HIP(hipStreamSynchronize(S->s));
+        }
+        if (f == CFD_F_LAP_SPARSE && S->ell_on) {
+            S->ell.refresh(S->m->t, S->LAP.p, S->s);
             CFD_HIP(hipStreamSynchronize(S->s));
         }
     })

@@ -524,11 +524,34 @@ struct PcgSystem {
     const int* x_fix_idx;
     int nfix;
 };
+// The matrix of a CSR system stored a second time in ELL order for the
+// SpMV kernels: entry k of row i (its k-th CSR entry) at k * n + i, column
+// 0-based; rows keep their CSR lengths (entries beyond are never read), so a
+// row's products are summed in the same order as from the CSR arrays.
+constexpr int kEllMaxWidth = 16;
+struct EllMat {
+    const double* val = nullptr;
+    const int* col = nullptr;
+    const int* rowptr = nullptr;
+    int width = 0;
+};
+struct EllLap {
+    int n = 0, width = 0;
+    DBuf<int> col;
+    DBuf<double> val;
+    // builds the column layout; false (nothing built) when the longest row
+    // exceeds kEllMaxWidth or 1.5 times the mean row length
+    bool build(const Tables& t, hipStream_t s);
+    // val from the CSR values A
+    void refresh(const Tables& t, const double* A, hipStream_t s);
+    EllMat mat(const Tables& t) const { return EllMat{val.p, col.p, t.rowptr.p, width}; }
+};
 // Several independent solves on one matrix, interleaved on stream s with
-// shared polls.
+// shared polls; ell (when given) is the same matrix in ELL order, used by
+// the SpMV kernels of the iterations (bit for bit the CSR kernels' results).
 void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A, const int* idx,
                       const int* rowptr, const double* diag, double tol, int maxit, int rel,
-                      hipStream_t s);
+                      hipStream_t s, const EllMat* ell = nullptr);
 // iterations of the last solve run by pcg_launch_multi on w, as its final
 // poll read them (host memory, no device access)
 double pcg_polled_iterations(const PcgWork& w);

@@ -201,92 +201,87 @@ __global__ void k_normalvel(const int* __restrict__ bcidx, const int* __restrict
 }
 
 // meshMove.f90:167-192 FORCES and ns2DComp.ALE.f90:839-887 FORCE_VISC, one
-// set.  The per-edge terms are formed in parallel into LDS; each running sum
-// keeps the reference's edge order on the first lane of its own wave (waves
-// 0-2: FX, FY, RM; waves 3-4: F_VX, F_VY), so the dependency chains overlap.
-// FORCE_VISC's traction is that of the full stress tensor (-p I + viscous,
-// with the element-average pressure), Sutherland mu at the edge-average
-// temperature, on the edge normal (Y2-Y1, -(X2-X1)) / |edge|.
+// set, in one block.  Thread t forms the terms of edges t, t + 1024, ... and
+// sums them in that order; the 1024 partials are then combined by a fixed
+// tree, so the result does not depend on scheduling (the reference sums in
+// edge order; the two orders differ at round-off).  FORCE_VISC's traction is
+// that of the full stress tensor (-p I + viscous, with the element-average
+// pressure), Sutherland mu at the edge-average temperature, on the edge
+// normal (Y2-Y1, -(X2-X1)) / |edge|.
+constexpr int kForceThreads = 1024;
 template <bool kPress, bool kVisc>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kForceThreads)
 k_forces(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
          const double* __restrict__ Y, const double* __restrict__ P, double xref, double yref,
          double* __restrict__ out, ViscIn v, double* __restrict__ outv) {
-    __shared__ double dfx[1024], dfy[1024], m1[1024], m2[1024], tvx[1024], tvy[1024];
-    double acc = 0.0;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int base = 0; base < nedge; base += 1024) {
-        int i = base + threadIdx.x;
-        if (i < nedge) {
-            int n1 = iset[3 * i] - 1, n2 = iset[3 * i + 1] - 1;
-            if (kPress) {
-                double d_press = (P[n1] + P[n2]) / 2.0;
-                double rlx = X[n1] - X[n2];
-                double rly = Y[n2] - Y[n1];
-                double fx = d_press * rly, fy = d_press * rlx;
-                double xc = (X[n1] + X[n2]) / 2.0;
-                double yc = (Y[n2] + Y[n1]) / 2.0;
-                dfx[threadIdx.x] = fx;
-                dfy[threadIdx.x] = fy;
-                m1[threadIdx.x] = fy * (xc - xref);
-                m2[threadIdx.x] = fx * (yc - yref);
-            }
-            if (kVisc) {
-                const int ie = iset[3 * i + 2] - 1;
-                const double temp = (v.T[n1] + v.T[n2]) / 2.0;
-                const double smu = 110.0;
-                const double fmu = 0.017 * pow_3_2(temp / v.t_inf) * (v.t_inf + smu) / (temp + smu);
-                double rly = -(X[n2] - X[n1]);
-                double rlx = Y[n2] - Y[n1];
-                const double rmod = sqrt(rlx * rlx + rly * rly);
-                rlx = rlx / rmod;
-                rly = rly / rmod;
-                double dux = 0.0, duy = 0.0, dvx = 0.0, dvy = 0.0, press = 0.0;
+    __shared__ double red[5][kForceThreads];
+    double fx = 0.0, fy = 0.0, rm = 0.0, fvx = 0.0, fvy = 0.0;
+    for (int i = threadIdx.x; i < nedge; i += kForceThreads) {
+        const int n1 = iset[3 * i] - 1, n2 = iset[3 * i + 1] - 1;
+        if (kPress) {
+            const double d_press = (P[n1] + P[n2]) / 2.0;
+            const double rlx = X[n1] - X[n2];
+            const double rly = Y[n2] - Y[n1];
+            const double dfx = d_press * rly, dfy = d_press * rlx;
+            const double xc = (X[n1] + X[n2]) / 2.0;
+            const double yc = (Y[n2] + Y[n1]) / 2.0;
+            fx = fx + dfx;
+            fy = fy + dfy;
+            rm = rm + dfy * (xc - xref) - dfx * (yc - yref);
+        }
+        if (kVisc) {
+            const int ie = iset[3 * i + 2] - 1;
+            const double temp = (v.T[n1] + v.T[n2]) / 2.0;
+            const double smu = 110.0;
+            const double fmu = 0.017 * pow_3_2(temp / v.t_inf) * (v.t_inf + smu) / (temp + smu);
+            double rly = -(X[n2] - X[n1]);
+            double rlx = Y[n2] - Y[n1];
+            const double rmod = sqrt(rlx * rlx + rly * rly);
+            rlx = rlx / rmod;
+            rly = rly / rmod;
+            double dux = 0.0, duy = 0.0, dvx = 0.0, dvy = 0.0, press = 0.0;
 #pragma unroll
-                for (int jj = 0; jj < 3; ++jj) {
-                    const int nn = v.inpoel[3 * ie + jj] - 1;
-                    const double nx = v.dNx[3 * ie + jj], ny = v.dNy[3 * ie + jj];
-                    const double vx = v.VX[nn], vy = v.VY[nn];
-                    dux = dux + nx * vx;
-                    duy = duy + ny * vx;
-                    dvx = dvx + nx * vy;
-                    dvy = dvy + ny * vy;
-                    press = press + P[nn];
-                }
-                press = press / 3.0;
-                const double txx = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dux);
-                const double txy = fmu * (duy + dvx);
-                const double tyy = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dvy);
-                tvx[threadIdx.x] = (txx * rlx + txy * rly) * rmod;
-                tvy[threadIdx.x] = (txy * rlx + tyy * rly) * rmod;
+            for (int jj = 0; jj < 3; ++jj) {
+                const int nn = v.inpoel[3 * ie + jj] - 1;
+                const double nx = v.dNx[3 * ie + jj], ny = v.dNy[3 * ie + jj];
+                const double vx = v.VX[nn], vy = v.VY[nn];
+                dux = dux + nx * vx;
+                duy = duy + ny * vx;
+                dvx = dvx + nx * vy;
+                dvy = dvy + ny * vy;
+                press = press + P[nn];
             }
+            press = press / 3.0;
+            const double txx = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dux);
+            const double txy = fmu * (duy + dvx);
+            const double tyy = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dvy);
+            fvx = fvx + (txx * rlx + txy * rly) * rmod;
+            fvy = fvy + (txy * rlx + tyy * rly) * rmod;
         }
-        __syncthreads();
-        if (lane == 0) {
-            // unrolled so the LDS reads run ahead of the (sequential) sums
-            const int m = nedge - base < 1024 ? nedge - base : 1024;
-            if (kPress && wave == 0) {
-#pragma unroll 16
-                for (int q = 0; q < m; ++q) acc = acc + dfx[q];
-            } else if (kPress && wave == 1) {
-#pragma unroll 16
-                for (int q = 0; q < m; ++q) acc = acc + dfy[q];
-            } else if (kPress && wave == 2) {  // RM = RM + DFY*(XC-XR) - DFX*(YC-YR)
-#pragma unroll 16
-                for (int q = 0; q < m; ++q) acc = acc + m1[q] - m2[q];
-            } else if (kVisc && wave == 3) {
-#pragma unroll 16
-                for (int q = 0; q < m; ++q) acc = acc + tvx[q];
-            } else if (kVisc && wave == 4) {
-#pragma unroll 16
-                for (int q = 0; q < m; ++q) acc = acc + tvy[q];
-            }
-        }
-        __syncthreads();
     }
-    if (lane == 0) {
-        if (kPress && wave < 3) out[wave] = acc;
-        if (kVisc && (wave == 3 || wave == 4)) outv[wave - 3] = acc;
+    const int t = threadIdx.x;
+    red[0][t] = fx;
+    red[1][t] = fy;
+    red[2][t] = rm;
+    red[3][t] = fvx;
+    red[4][t] = fvy;
+    for (int h = kForceThreads / 2; h > 0; h >>= 1) {
+        __syncthreads();
+        if (t < h) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) red[q][t] = red[q][t] + red[q][t + h];
+        }
+    }
+    if (t == 0) {
+        if (kPress) {
+            out[0] = red[0][0];
+            out[1] = red[1][0];
+            out[2] = red[2][0];
+        }
+        if (kVisc) {
+            outv[0] = red[3][0];
+            outv[1] = red[4][0];
+        }
     }
 }
 
@@ -363,11 +358,14 @@ void launch_forces(int nedge, const int* iset, const double* X, const double* Y,
     if (nedge == 0 || (!out3 && !visc)) return;
     const ViscIn v = visc ? *visc : ViscIn{};
     if (out3 && visc)
-        k_forces<true, true><<<1, 1024, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3, v, out2);
+        k_forces<true, true><<<1, kForceThreads, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3, v,
+                                                          out2);
     else if (out3)
-        k_forces<true, false><<<1, 1024, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3, v, out2);
+        k_forces<true, false><<<1, kForceThreads, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3,
+                                                           v, out2);
     else
-        k_forces<false, true><<<1, 1024, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3, v, out2);
+        k_forces<false, true><<<1, kForceThreads, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3,
+                                                           v, out2);
     CFD_HIP(hipGetLastError());
 }
 

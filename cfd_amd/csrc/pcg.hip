@@ -386,6 +386,109 @@ k_fin_b2(PairSys s0, PairSys s1, int nb) {
     }
 }
 
+// ---- ELL-order SpMV variants of k_iter_b / k_iter_b2: the same rows, the
+// same per-row summation order and block decomposition, the entries read
+// coalesced from the ELL copy (no LDS staging, no barriers before the
+// block partial)
+template <int kW>
+__device__ inline double ell_row_dot(const EllMat& m, int n, int i, int len, double beta,
+                                     const double* __restrict__ pold, const double* __restrict__ z,
+                                     bool first) {
+    double dot = 0.0;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        if (k < len) {
+            const size_t q = (size_t)k * n + i;
+            const double a = m.val[q];
+            const int c = m.col[q];
+            const double pc = first ? z[c] : beta * pold[c] + z[c];
+            dot = dot + a * pc;
+        }
+    }
+    return dot;
+}
+
+template <int kW>
+__global__ void __launch_bounds__(kBlock)
+k_iter_b_ell(int n, EllMat m, const int* __restrict__ fixflag, const double* __restrict__ pold,
+             const double* __restrict__ z, double* __restrict__ pnew, double* __restrict__ y,
+             int first, const double* __restrict__ scal, double* __restrict__ partials) {
+    if (scal[S_DONE] != 0.0 || scal[S_STOP] != 0.0) return;
+    const double beta = scal[S_BETA];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const bool row = !fixflag[i];
+        const double dot =
+            row ? ell_row_dot<kW>(m, n, i, m.rowptr[i + 1] - m.rowptr[i], beta, pold, z, first)
+                : 0.0;
+        const double pi = first ? z[i] : beta * pold[i] + z[i];
+        const double yi = row ? dot : 1.0e30 * pi;
+        pnew[i] = pi;
+        y[i] = yi;
+        v = pi * yi;
+    }
+    block_partial(v, OpSum(), partials);
+}
+
+template <int kW>
+__global__ void __launch_bounds__(kBlock)
+k_iter_b2_ell(int n, EllMat m, PairSys s0, PairSys s1) {
+    const bool on0 = s0.scal[S_DONE] == 0.0 && s0.scal[S_STOP] == 0.0;
+    const bool on1 = s1.scal[S_DONE] == 0.0 && s1.scal[S_STOP] == 0.0;
+    if (!on0 && !on1) return;
+    const double beta0 = s0.scal[S_BETA], beta1 = s1.scal[S_BETA];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);  // logical row block
+    const int i = lb * blockDim.x + threadIdx.x;
+    const bool act = i < n;
+    const bool row0 = act && !s0.fixflag[i], row1 = act && !s1.fixflag[i];
+    double dot0 = 0.0, dot1 = 0.0;
+    if ((on0 && row0) || (on1 && row1)) {
+        const int len = m.rowptr[i + 1] - m.rowptr[i];
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            if (k < len) {
+                const size_t q = (size_t)k * n + i;
+                const double a = m.val[q];
+                const int c = m.col[q];
+                if (on0 && row0) dot0 = dot0 + a * (beta0 * s0.pold[c] + s0.z[c]);
+                if (on1 && row1) dot1 = dot1 + a * (beta1 * s1.pold[c] + s1.z[c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const PairSys& q = k ? s1 : s0;
+        if (!(k ? on1 : on0)) continue;  // uniform over the grid
+        double v = 0.0;
+        if (act) {
+            const double pi = (k ? beta1 : beta0) * q.pold[i] + q.z[i];
+            const double yi = (k ? row1 : row0) ? (k ? dot1 : dot0) : 1.0e30 * pi;
+            q.pnew[i] = pi;
+            q.y[i] = yi;
+            v = pi * yi;
+        }
+        block_partial(v, OpSum(), q.partial, lb);
+    }
+}
+
+// ELL column layout and values (EllLap::build / refresh)
+__global__ void k_ell_col(int n, int width, const int* __restrict__ rowptr,
+                          const int* __restrict__ idx, int* __restrict__ col) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = rowptr[i], len = rowptr[i + 1] - b;
+    for (int k = 0; k < width; ++k) col[(size_t)k * n + i] = k < len ? idx[b + k] - 1 : i;
+}
+
+__global__ void k_ell_val(int n, int width, const int* __restrict__ rowptr,
+                          const double* __restrict__ A, double* __restrict__ val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = rowptr[i], len = rowptr[i + 1] - b;
+    for (int k = 0; k < width; ++k) val[(size_t)k * n + i] = k < len ? A[b + k] : 0.0;
+}
+
 __global__ void k_store_result(const double* __restrict__ scal, double* __restrict__ iters,
                                double* __restrict__ rz, double* __restrict__ itsum,
                                double* __restrict__ unconv) {
@@ -424,6 +527,30 @@ void pcg_store_result(const PcgWork& w, double* iters, double* rz, double* itsum
     CFD_HIP(hipGetLastError());
 }
 
+bool EllLap::build(const Tables& t, hipStream_t s) {
+    n = t.npoin;
+    width = 0;
+    if (n <= 0 || t.nnz <= 0) return false;
+    std::vector<int> rp(n + 1);  // the longest row, on the host (once per session)
+    t.rowptr.download(rp.data(), n + 1, s);
+    CFD_HIP(hipStreamSynchronize(s));
+    int w = 0;
+    for (int i = 0; i < n; ++i) w = std::max(w, rp[i + 1] - rp[i]);
+    if (w <= 0 || w > kEllMaxWidth || (double)w * n > 1.5 * (double)t.nnz) return false;
+    width = w;
+    col.alloc((size_t)w * n);
+    val.alloc((size_t)w * n);
+    k_ell_col<<<grid_for(n), kBlock, 0, s>>>(n, w, t.rowptr.p, t.idx.p, col.p);
+    CFD_HIP(hipGetLastError());
+    return true;
+}
+
+void EllLap::refresh(const Tables& t, const double* A, hipStream_t s) {
+    if (!width) return;
+    k_ell_val<<<grid_for(n), kBlock, 0, s>>>(n, width, t.rowptr.p, A, val.p);
+    CFD_HIP(hipGetLastError());
+}
+
 void PcgWork::ensure(int npoin) {
     if (npoin == n && r.p) return;
     n = npoin;
@@ -439,9 +566,16 @@ void PcgWork::ensure(int npoin) {
     hpin.alloc(S_NSCAL);
 }
 
+// launches the ELL SpMV kernel of the width class that holds m.width
+template <class F8, class F16>
+static void ell_dispatch(const EllMat& m, F8 f8, F16 f16) {
+    if (m.width <= 8) f8();
+    else f16();
+}
+
 void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A, const int* idx,
                       const int* rowptr, const double* diag, double tol, int maxit, int rel,
-                      hipStream_t s) {
+                      hipStream_t s, const EllMat* ell) {
     CFD_CHECK(npoin > 0, "arg: PCG needs npoin > 0");
     constexpr int kMaxSys = 2;
     CFD_CHECK(nsys >= 1 && nsys <= kMaxSys, "internal: PCG systems per launch");
@@ -459,8 +593,24 @@ void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A
     auto iter_b = [&](int k, int first) {
         PcgWork& w = *sys[k].w;
         double* pbuf[2] = {w.pa.p, w.pb.p};
-        k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[cur[k]], w.z.p,
-                                   pbuf[cur[k] ^ 1], w.y.p, first, w.scal.p, w.partial.p);
+        if (ell) {
+            const EllMat m = *ell;
+            ell_dispatch(
+                m,
+                [&] {
+                    k_iter_b_ell<8><<<g, blk, 0, s>>>(npoin, m, w.fixflag.p, pbuf[cur[k]], w.z.p,
+                                                      pbuf[cur[k] ^ 1], w.y.p, first, w.scal.p,
+                                                      w.partial.p);
+                },
+                [&] {
+                    k_iter_b_ell<kEllMaxWidth><<<g, blk, 0, s>>>(
+                        npoin, m, w.fixflag.p, pbuf[cur[k]], w.z.p, pbuf[cur[k] ^ 1], w.y.p, first,
+                        w.scal.p, w.partial.p);
+                });
+        } else {
+            k_iter_b<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, w.fixflag.p, pbuf[cur[k]], w.z.p,
+                                       pbuf[cur[k] ^ 1], w.y.p, first, w.scal.p, w.partial.p);
+        }
         k_fin_b<<<one, fin, 0, s>>>(w.partial.p, nb, w.scal.p);
     };
     for (int k = 0; k < nsys; ++k) {
@@ -515,7 +665,14 @@ void pcg_launch_multi(const PcgSystem* sys, int nsys, int npoin, const double* A
                 const PairSys p0 = pair(0), p1 = pair(1);
                 k_iter_a2<<<g, blk, 0, s>>>(npoin, p0, p1, diag);
                 k_fin_a2<<<dim3(2), fin, 0, s>>>(p0, p1, nb);
-                k_iter_b2<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, p0, p1);
+                if (ell) {
+                    const EllMat m = *ell;
+                    ell_dispatch(
+                        m, [&] { k_iter_b2_ell<8><<<g, blk, 0, s>>>(npoin, m, p0, p1); },
+                        [&] { k_iter_b2_ell<kEllMaxWidth><<<g, blk, 0, s>>>(npoin, m, p0, p1); });
+                } else {
+                    k_iter_b2<<<g, blk, 0, s>>>(npoin, A, idx, rowptr, p0, p1);
+                }
                 k_fin_b2<<<dim3(2), fin, 0, s>>>(p0, p1, nb);
                 cur[0] ^= 1;
                 cur[1] ^= 1;
