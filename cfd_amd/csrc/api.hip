@@ -641,6 +641,11 @@ struct cfd_session {
     // next step skips the learned candidate (and its training), as nothing
     // can be gained.  The relative rule always uses it.
     bool last_solves_trivial = false;
+    // With ITLOCAL = 0 the fused paths skip DT's clip and the DTL fill (DTL
+    // would be DTMIN everywhere, and no kernel of theirs reads DT or DTL);
+    // the two fields are formed when a caller reads them (dt_stale).
+    bool dt_stale = false;
+    bool lazy_dt() const { return fused() && d.itlocal == 0; }
     // timing
     bool timing = false;
     unsigned tmask = 0;
@@ -723,7 +728,7 @@ struct cfd_session {
         o.tc = tc;
         o.fac = fac;
         o.area = AREA.p;
-        o.dtl = DTL.p;
+        o.dtl = lazy_dt() ? nullptr : DTL.p;
         o.inv_perm = patches.inv_perm.p;
         o.cte = c.cte;
         return o;
@@ -806,11 +811,19 @@ struct cfd_session {
         tb(CFD_K_STEP);
         // deltat, DTMIN with its hysteresis, DTL, the clock (ns2DComp.ALE.f90:142-166)
         tb(CFD_K_DELTAT);
-        const int nb = launch_deltat_partials(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p, DT.p,
-                                              partials.p, s);
+        // fused paths: in patch order with the patch's nodes in LDS (the same DT)
+        const int nb = fused() && patches.npatch <= (int)partials.n
+                           ? launch_deltat_patch(patches, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p,
+                                                 DT.p, partials.p, s)
+                           : launch_deltat_partials(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p,
+                                                    DT.p, partials.p, s);
         launch_step_ctl(partials.p, nullptr, nb, ctl.p, 1, d.itlocal, d.motion, d.motion_amp,
                         d.motion_omega, s);
-        launch_dt_finish(DT.p, DTL.p, ctl.p, nullptr, d.itlocal != 0, ne, s);
+        if (lazy_dt()) {
+            dt_stale = true;
+        } else {
+            launch_dt_finish(DT.p, DTL.p, ctl.p, nullptr, d.itlocal != 0, ne, s);
+        }
         te(CFD_K_DELTAT);
         const double* dtmin = scal(CFD_S_DTMIN);
 
@@ -1269,6 +1282,10 @@ int cfd_session_get_field(cfd_session* S, int f, double* host_out) {
         if (S->d.precision == 0 && estab_field)  // from the fused FP64 path's patch order
             launch_estab_widen(S->elem64(), S->patches.inv_perm.p, S->ne, S->SHOC.p, S->T1.p,
                                S->T2.p, S->T3.p, S->s);
+        if ((f == CFD_F_DT || f == CFD_F_DTL) && S->dt_stale) {  // the last step's clip and DTL
+            launch_dt_finish(S->DT.p, S->DTL.p, S->ctl.p, nullptr, false, S->ne, S->s);
+            S->dt_stale = false;
+        }
         b->download(host, n, S->s);
         CFD_HIP(hipStreamSynchronize(S->s));
         if (S->rn) renumber_field(*S->rn, f, tmp.data(), host_out, true);
@@ -1294,6 +1311,7 @@ int cfd_session_set_field(cfd_session* S, int f, const double* host_in) {
             S->refresh_patch_geometry();
             CFD_HIP(hipStreamSynchronize(S->s));
         }
+        if (f == CFD_F_DT || f == CFD_F_DTL) S->dt_stale = false;  // the caller's values
         if (f == CFD_F_LAP_SPARSE && S->ell_on) {
             S->ell.refresh(S->m->t, S->LAP.p, S->s);
             CFD_HIP(hipStreamSynchronize(S->s));

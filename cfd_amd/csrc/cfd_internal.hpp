@@ -207,6 +207,12 @@ constexpr int kRedBlocks = 2048;
 int launch_deltat_partials(const Tables& t, const Consts& c, const double* area, const double* T,
                            const double* VX, const double* VY, const double* WX, const double* WY,
                            double* DT, double* partials, hipStream_t s);
+// The same in patch order (one block per patch of ps, its node values in
+// LDS): the same DT bits, one min partial per patch; returns ps.npatch
+struct PatchSet;
+int launch_deltat_patch(const PatchSet& ps, const Consts& c, const double* area, const double* T,
+                        const double* VX, const double* VY, const double* WX, const double* WY,
+                        double* DT, double* partials, hipStream_t s);
 // The same with block b over the elements [chunk[b], chunk[b+1]): one min
 // partial per chunk (batches keep each chunk inside one case)
 void launch_deltat_chunks(const Tables& t, const Consts& c, const double* area, const double* T,
@@ -221,7 +227,7 @@ void launch_deltat_chunks(const Tables& t, const Consts& c, const double* area, 
 template <class R>
 struct ElemP {
     R *nu = nullptr, *ta = nullptr, *tb = nullptr, *tc = nullptr, *fac = nullptr;
-    const double *area = nullptr, *dtl = nullptr;
+    const double *area = nullptr, *dtl = nullptr;  // dtl null: DTL = DTMIN everywhere
     const int* inv_perm = nullptr;  // element e's patch position
     double cte = 1.0;
     // batches: element e belongs to case ecase[e]; its DTMIN is

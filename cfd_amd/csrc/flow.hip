@@ -26,6 +26,30 @@ namespace {
 // ---------------------------------------------------------------------------
 // deltat
 // ---------------------------------------------------------------------------
+// One element's time step (subrutinas.f90:178-207) from its nodes' T and
+// the magnitudes of their velocity components relative to the mesh
+__device__ __forceinline__ double deltat_elem(const double (&Tn)[3], const double (&vu)[3],
+                                              const double (&vv)[3], double area, double t_inf,
+                                              double fsafe) {
+    double vumax = 0.0, vvmax = 0.0;
+    const double t_iel = (Tn[0] + Tn[1] + Tn[2]) / 3.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (vu[i] > vumax) vumax = vu[i];
+        if (vv[i] > vvmax) vvmax = vv[i];
+    }
+    const double hh = sqrt(2.0 * area);
+    const double vel = sqrt(vumax * vumax + vvmax * vvmax);
+    const double smu = 110.0;
+    const double fmu = 0.017 * pow_3_2(t_iel / t_inf) * (t_inf + smu) / (t_iel + smu);
+    const double et = fmu;
+    const double pe = (vel * hh) / (2.0 * et);
+    const double alpha = pe / 3.0 < 1.0 ? pe / 3.0 : 1.0;
+    const double deltatu = 1.0 / (4.0 * et / (hh * hh) + alpha * vel / hh);
+    const double deltatc = 1.0 / (4.0 * et / (hh * hh));
+    return fsafe / (1.0 / deltatc + 1.0 / deltatu);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_deltat(const int* __restrict__ inpoel, const double* __restrict__ area, const double* __restrict__ T,
          const double* __restrict__ VX, const double* __restrict__ VY, const double* __restrict__ WX,
@@ -49,32 +73,65 @@ k_deltat(const int* __restrict__ inpoel, const double* __restrict__ area, const 
     double dmin = 1.0e20;
     for (int e = beg; e < end; e += step) {
         const int* ip = inpoel + 3 * e;
-        double vumax = 0.0, vvmax = 0.0;
-        int p0 = ip[0] - 1, p1 = ip[1] - 1, p2 = ip[2] - 1;
-        double t_iel = (T[p0] + T[p1] + T[p2]) / 3.0;
-        int pp[3] = {p0, p1, p2};
+        const int pp[3] = {ip[0] - 1, ip[1] - 1, ip[2] - 1};
+        double Tn[3], vu[3], vv[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            int p = pp[i];
-            double vu = fabs(VX[p] - WX[p]);
-            double vv = fabs(VY[p] - WY[p]);
-            if (vu > vumax) vumax = vu;
-            if (vv > vvmax) vvmax = vv;
+            Tn[i] = T[pp[i]];
+            vu[i] = fabs(VX[pp[i]] - WX[pp[i]]);
+            vv[i] = fabs(VY[pp[i]] - WY[pp[i]]);
         }
-        double hh = sqrt(2.0 * area[e]);
-        double vel = sqrt(vumax * vumax + vvmax * vvmax);
-        double smu = 110.0;
-        double fmu = 0.017 * pow_3_2(t_iel / t_inf) * (t_inf + smu) / (t_iel + smu);
-        double et = fmu;
-        double pe = (vel * hh) / (2.0 * et);
-        double alpha = pe / 3.0 < 1.0 ? pe / 3.0 : 1.0;
-        double deltatu = 1.0 / (4.0 * et / (hh * hh) + alpha * vel / hh);
-        double deltatc = 1.0 / (4.0 * et / (hh * hh));
-        double dtelem = fsafe / (1.0 / deltatc + 1.0 / deltatu);
+        const double dtelem = deltat_elem(Tn, vu, vv, area[e], t_inf, fsafe);
         DT[e] = dtelem;
         if (dtelem < dmin) dmin = dtelem;
     }
     block_partial(dmin, OpMin(), partials, lb);
+}
+
+// deltat in patch order: one block per patch, the patch's node values (T,
+// |VX - WX|, |VY - WY|) staged in LDS, each element's DT from deltat_elem
+// (the same values as k_deltat), one min partial per patch
+struct DtNode {
+    double T, vu, vv;
+};
+__global__ void __launch_bounds__(kBlock)
+k_deltat_patch(const int* __restrict__ node_base, const int* __restrict__ elem_base,
+               const int* __restrict__ pnode, const uint32_t* __restrict__ loc,
+               const int* __restrict__ perm, int npatch, const double* __restrict__ area,
+               const double* __restrict__ T, const double* __restrict__ VX,
+               const double* __restrict__ VY, const double* __restrict__ WX,
+               const double* __restrict__ WY, double* __restrict__ DT, double t_inf, double fsafe,
+               double* __restrict__ partials) {
+    extern __shared__ double4 smem4[];
+    DtNode* sN = reinterpret_cast<DtNode*>(smem4);
+    const int p = xcd_block(blockIdx.x, npatch);  // grid = npatch: logical patch
+    const int nb = node_base[p], nl = node_base[p + 1] - nb;
+    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+        const int g = pnode[nb + l] - 1;
+        sN[l] = DtNode{T[g], fabs(VX[g] - WX[g]), fabs(VY[g] - WY[g])};
+    }
+    __syncthreads();
+    const long long e0 = elem_base[p];
+    const int nep = elem_base[p + 1] - elem_base[p];
+    double dmin = 1.0e20;
+    for (int q = threadIdx.x; q < nep; q += blockDim.x) {
+        const long long e = e0 + q;
+        const uint64_t w = (uint64_t)loc[2 * e] | ((uint64_t)loc[2 * e + 1] << 32);
+        const int ln[3] = {(int)(w & 1023u), (int)((w >> 10) & 1023u), (int)((w >> 20) & 1023u)};
+        double Tn[3], vu[3], vv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const DtNode& nd = sN[ln[i]];
+            Tn[i] = nd.T;
+            vu[i] = nd.vu;
+            vv[i] = nd.vv;
+        }
+        const int eo = perm[e];
+        const double dtelem = deltat_elem(Tn, vu, vv, area[eo], t_inf, fsafe);
+        DT[eo] = dtelem;
+        if (dtelem < dmin) dmin = dtelem;
+    }
+    block_partial(dmin, OpMin(), partials, p);
 }
 
 __global__ void k_deltat_clip(double* __restrict__ DT, const double* __restrict__ dtmin, int nelem) {
@@ -941,7 +998,8 @@ k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
             o.ta[e] = (R)rrr;
             o.tb[e] = (R)s2;
             o.tc[e] = (R)s3;
-            o.fac[e] = (R)(o.area[eo] * o.dtl[eo] / 3.0);
+            // DTL = DTMIN in every element unless local time steps blend in DT
+            o.fac[e] = (R)(o.area[eo] * (o.dtl ? o.dtl[eo] : dtmin) / 3.0);
         }
         __syncthreads();  // the next patch overwrites sN
     }
@@ -969,6 +1027,18 @@ int launch_deltat_partials(const Tables& t, const Consts& c, const double* area,
                                   partials, nullptr, nullptr, nullptr);
     CFD_HIP(hipGetLastError());
     return (int)g;
+}
+
+int launch_deltat_patch(const PatchSet& ps, const Consts& c, const double* area, const double* T,
+                        const double* VX, const double* VY, const double* WX, const double* WY,
+                        double* DT, double* partials, hipStream_t s) {
+    CFD_CHECK(ps.npatch > 0 && ps.max_nodes <= kPatchMaxNodes, "internal: deltat patches");
+    const size_t lds = (size_t)ps.max_nodes * sizeof(DtNode);
+    k_deltat_patch<<<ps.npatch, kBlock, lds, s>>>(ps.node_base.p, ps.elem_base.p, ps.pnode.p,
+                                                  ps.loc.p, ps.perm.p, ps.npatch, area, T, VX, VY,
+                                                  WX, WY, DT, c.t_inf, c.fsafe, partials);
+    CFD_HIP(hipGetLastError());
+    return ps.npatch;
 }
 
 void launch_deltat_chunks(const Tables& t, const Consts& c, const double* area, const double* T,
