@@ -321,6 +321,41 @@ module cfd_abi
             type(c_ptr), value :: s
             real(c_double), intent(in) :: scalars(*)
         end function
+        ! restart file of a session (ns2DComp.ALE.f90:394-435, :898-917); path ends
+        ! with c_null_char
+        integer(c_int) function cfd_session_write_restart(s, path) &
+            bind(c, name='cfd_session_write_restart')
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: s
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+        integer(c_int) function cfd_session_read_restart(s, path, keep_clock) &
+            bind(c, name='cfd_session_read_restart')
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: s
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), value :: keep_clock
+        end function
+
+        ! ---- restart files, host only (PRINTREST ns2DComp.ALE.f90:898-917,
+        ! RESTART :394-435); path ends with c_null_char ----
+        integer(c_int) function cfd_write_restart(path, iter, time, npoin, u, t, gamm) &
+            bind(c, name='cfd_write_restart')
+            import :: c_int, c_char, c_double
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), value :: iter, npoin
+            real(c_double), value :: time
+            real(c_double), intent(in) :: u(*), t(*), gamm(*)
+        end function
+        integer(c_int) function cfd_read_restart(path, npoin, iter, time, u, t, gamm) &
+            bind(c, name='cfd_read_restart')
+            import :: c_int, c_char, c_double
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), value :: npoin
+            integer(c_int), intent(out) :: iter
+            real(c_double), intent(out) :: time
+            real(c_double), intent(out) :: u(*), t(*), gamm(*)
+        end function
 
         ! ---- batches of independent fixed-mesh cases ----
         integer(c_int) function cfd_batch_create(b, ncases, cases, precision, itlocal, &

@@ -122,3 +122,53 @@ def test_session_checkpoint_through_a_restart_file(tmp_path):
     assert np.all(relnorm_cols(got["RHS"], ref["RHS"]) < 1e-12)
     A.close()
     B.close()
+
+
+_FORTRAN_RESTART = """\
+program restart_io
+    use iso_c_binding
+    use cfd_abi
+    implicit none
+    integer, parameter :: n = 7
+    real(c_double) :: u(4, n), t(n), g(n), u2(4, n), t2(n), g2(n), tm
+    integer(c_int) :: i, k, it
+    do i = 1, n
+        do k = 1, 4
+            u(k, i) = i + k / 8.0d0
+        end do
+        t(i) = 280.0d0 + i / 3.0d0
+        g(i) = 1.4d0
+    end do
+    call cfd_check(cfd_write_restart('f.RST' // c_null_char, 42_c_int, 0.375d0, n, u, t, g), &
+                   'write')
+    call cfd_check(cfd_read_restart('f.RST' // c_null_char, n, it, tm, u2, t2, g2), 'read')
+    if (it /= 42 .or. tm /= 0.375d0 .or. any(u2 /= u) .or. any(t2 /= t) .or. any(g2 /= g)) &
+        error stop 'restart round trip differs'
+    print '(a)', 'ok'
+end program restart_io
+"""
+
+
+def test_fortran_interface_writes_the_restart_file(tmp_path):
+    """A Fortran caller writes and reads a restart file through the
+    iso_c_binding interfaces of cfd_amd/fortran/cfd_abi.f90 (host-only
+    entries: no GPU); its file is byte for byte the Python binding's."""
+    import shutil
+    import subprocess
+    from cfd_amd import build as B
+    abi_o = os.path.join(B.OBJ, "cfd_abi_f.o")
+    if shutil.which("flang") is None or not os.path.isfile(abi_o):
+        pytest.skip("needs flang and the built Fortran interface module")
+    src = tmp_path / "restart_io.f90"
+    src.write_text(_FORTRAN_RESTART)
+    exe = str(tmp_path / "restart_io")
+    subprocess.run(["flang", "-O0", "-I", B.OBJ, "-c", str(src), "-o", exe + ".o"],
+                   check=True, capture_output=True, cwd=tmp_path)
+    subprocess.run(["flang", exe + ".o", abi_o, "-L" + B.PKG, "-lcfd_amd",
+                    "-Wl,-rpath," + B.PKG, "-o", exe], check=True, capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
+    i = np.arange(1, 8, dtype=np.float64)
+    U = i[:, None] + np.arange(1, 5) / 8.0
+    write_restart(str(tmp_path / "py.RST"), 42, 0.375, U, 280.0 + i / 3.0, np.full(7, 1.4))
+    assert (tmp_path / "f.RST").read_bytes() == (tmp_path / "py.RST").read_bytes()
