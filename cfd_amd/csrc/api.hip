@@ -42,10 +42,6 @@ __global__ void k_move(double* __restrict__ X, double* __restrict__ X1, double* 
     W[i] = d / dtmin;
 }
 
-__global__ void k_put2(double* __restrict__ a, double va, double* __restrict__ b, double vb) {
-    *a = va;
-    *b = vb;
-}
 }  // namespace
 
 }  // namespace cfd
@@ -366,7 +362,8 @@ int cfd_force_visc(cfd_mesh* m, int nedge, const int* iset, const double* X, con
         double* dX = m->put("X", X, np);
         double* dY = m->put("Y", Y, np);
         double* dP = m->put("P", P, np);
-        launch_forces(nedge, is, dX, dY, dP, 0.0, 0.0, nullptr, &v, m->dscal.p, m->s);
+        launch_forces(nedge, is, dX, dY, dP, 0.0, 0.0, nullptr, &v, m->dscal.p,
+                      m->scratch("fpart", forces_part_size(nedge)), nullptr, m->s);
         double out[2];
         m->get(out, (const double*)m->dscal.p, 2);
         m->sync();
@@ -622,6 +619,7 @@ struct cfd_session {
     DBuf<double> DNX64, DNY64, NU64, TA64, TB64, TC64, FAC64;
     PatchSet patches;
     DBuf<int> ILAUX, ISET;
+    DBuf<double> FPART;  // the forces' block partials
     BcDevice bc;
     PcgWork pcg, pcg_y;  // the x and y mesh solves
     EllLap ell;          // the Laplace values again in ELL order, for the solves' SpMV
@@ -881,14 +879,16 @@ struct cfd_session {
         const ViscIn visc{t.inpoel.p, T.p, VX.p, VY.p, DNX.p, DNY.p, c.t_inf};
         const bool fvisc = c.fmu != 0.0;
         const bool still = d.nmove == 0 && xpos_zero && move_normalized;
+        // when nothing moves (see xpos_zero) both solves return at once: their
+        // iteration counts and residuals (PCG_ITX, _ITY, _RZX, _RZY) are zeroed
+        // by the forces' launch
+        static_assert(CFD_S_PCG_ITY == CFD_S_PCG_ITX + 1 && CFD_S_PCG_RZX == CFD_S_PCG_ITX + 2 &&
+                          CFD_S_PCG_RZY == CFD_S_PCG_ITX + 3,
+                      "mesh-solve scalars are contiguous");
         launch_forces(d.nsetedges, ISET.p, X.p, Y.p, P.p, d.xref, d.yref, scal(CFD_S_FX),
-                      fvisc && still ? &visc : nullptr, scal(CFD_S_FVX), s);
-        if (still) {
-            // both solves return at once: nothing moves (see xpos_zero)
-            k_put2<<<1, 1, 0, s>>>(scal(CFD_S_PCG_ITX), 0.0, scal(CFD_S_PCG_RZX), 0.0);
-            k_put2<<<1, 1, 0, s>>>(scal(CFD_S_PCG_ITY), 0.0, scal(CFD_S_PCG_RZY), 0.0);
-            CFD_HIP(hipGetLastError());
-        } else {
+                      fvisc && still ? &visc : nullptr, scal(CFD_S_FVX), FPART.p,
+                      still ? scal(CFD_S_PCG_ITX) : nullptr, s);
+        if (!still) {
             // TRANSF's displacements reach the solves only through moving nodes
             if (d.nmove > 0)
                 launch_transf(d.nsetedges, ISET.p, X.p, Y.p, scal(X_CA), 0.0, d.xref, d.yref,
@@ -897,7 +897,7 @@ struct cfd_session {
             move_normalized = true;
             if (fvisc)
                 launch_forces(d.nsetedges, ISET.p, X.p, Y.p, P.p, d.xref, d.yref, nullptr, &visc,
-                              scal(CFD_S_FVX), s);
+                              scal(CFD_S_FVX), FPART.p, nullptr, s);
         }
 
         if (d.moving == 1) {  // ns2DComp.ALE.f90:259-275
@@ -1150,6 +1150,7 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->ILAUX.upload(S->ilaux_h.data(), nn, s);
         S->ISET.alloc(std::max<size_t>(3, S->iset_h.size()));
         S->ISET.upload(S->iset_h.data(), S->iset_h.size(), s);
+        S->FPART.alloc(forces_part_size((int)(S->iset_h.size() / 3)));
         S->ctl.alloc(X_NCTL);
         S->ctl.zero(s);
         S->partials.alloc(grid_for(std::max(ne, np)));

@@ -501,9 +501,12 @@ struct ViscIn {
     const double *T, *VX, *VY, *dNx, *dNy;
     double t_inf;
 };
+// part: device scratch of forces_part_size(nedge) doubles; zero4 (may be
+// null): four device scalars set to 0 in the same launches
+size_t forces_part_size(int nedge);
 void launch_forces(int nedge, const int* iset, const double* X, const double* Y, const double* P,
                    double xref, double yref, double* out3, const ViscIn* visc, double* out2,
-                   hipStream_t s);
+                   double* part, double* zero4, hipStream_t s);
 // cs: device scalars cos(alpha), sin(alpha)
 void launch_transf(int nedge, const int* iset, const double* X, const double* Y, const double* cs,
                    double yposr, double xref, double yref, double* DXPOS, double* DYPOS,

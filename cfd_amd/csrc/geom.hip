@@ -201,88 +201,104 @@ __global__ void k_normalvel(const int* __restrict__ bcidx, const int* __restrict
 }
 
 // meshMove.f90:167-192 FORCES and ns2DComp.ALE.f90:839-887 FORCE_VISC, one
-// set, in one block.  Thread t forms the terms of edges t, t + 1024, ... and
-// sums them in that order; the 1024 partials are then combined by a fixed
-// tree, so the result does not depend on scheduling (the reference sums in
-// edge order; the two orders differ at round-off).  FORCE_VISC's traction is
-// that of the full stress tensor (-p I + viscous, with the element-average
-// pressure), Sutherland mu at the edge-average temperature, on the edge
-// normal (Y2-Y1, -(X2-X1)) / |edge|.
-constexpr int kForceThreads = 1024;
+// set.  FORCE_VISC's traction is that of the full stress tensor (-p I +
+// viscous, with the element-average pressure), Sutherland mu at the
+// edge-average temperature, on the edge normal (Y2-Y1, -(X2-X1)) / |edge|.
+// r = FX, FY, RM, F_VX, F_VY terms of edge i.
 template <bool kPress, bool kVisc>
-__global__ void __launch_bounds__(kForceThreads)
-k_forces(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
-         const double* __restrict__ Y, const double* __restrict__ P, double xref, double yref,
-         double* __restrict__ out, ViscIn v, double* __restrict__ outv) {
-    __shared__ double red[5][kForceThreads];
-    double fx = 0.0, fy = 0.0, rm = 0.0, fvx = 0.0, fvy = 0.0;
-    for (int i = threadIdx.x; i < nedge; i += kForceThreads) {
-        const int n1 = iset[3 * i] - 1, n2 = iset[3 * i + 1] - 1;
-        if (kPress) {
-            const double d_press = (P[n1] + P[n2]) / 2.0;
-            const double rlx = X[n1] - X[n2];
-            const double rly = Y[n2] - Y[n1];
-            const double dfx = d_press * rly, dfy = d_press * rlx;
-            const double xc = (X[n1] + X[n2]) / 2.0;
-            const double yc = (Y[n2] + Y[n1]) / 2.0;
-            fx = fx + dfx;
-            fy = fy + dfy;
-            rm = rm + dfy * (xc - xref) - dfx * (yc - yref);
-        }
-        if (kVisc) {
-            const int ie = iset[3 * i + 2] - 1;
-            const double temp = (v.T[n1] + v.T[n2]) / 2.0;
-            const double smu = 110.0;
-            const double fmu = 0.017 * pow_3_2(temp / v.t_inf) * (v.t_inf + smu) / (temp + smu);
-            double rly = -(X[n2] - X[n1]);
-            double rlx = Y[n2] - Y[n1];
-            const double rmod = sqrt(rlx * rlx + rly * rly);
-            rlx = rlx / rmod;
-            rly = rly / rmod;
-            double dux = 0.0, duy = 0.0, dvx = 0.0, dvy = 0.0, press = 0.0;
+__device__ __forceinline__ void force_terms(int i, const int* __restrict__ iset,
+                                            const double* __restrict__ X,
+                                            const double* __restrict__ Y,
+                                            const double* __restrict__ P, double xref,
+                                            double yref, const ViscIn& v, double r[5]) {
+    const int n1 = iset[3 * i] - 1, n2 = iset[3 * i + 1] - 1;
+    if (kPress) {
+        const double d_press = (P[n1] + P[n2]) / 2.0;
+        const double rlx = X[n1] - X[n2];
+        const double rly = Y[n2] - Y[n1];
+        const double dfx = d_press * rly, dfy = d_press * rlx;
+        const double xc = (X[n1] + X[n2]) / 2.0;
+        const double yc = (Y[n2] + Y[n1]) / 2.0;
+        r[0] = dfx;
+        r[1] = dfy;
+        r[2] = dfy * (xc - xref) - dfx * (yc - yref);
+    }
+    if (kVisc) {
+        const int ie = iset[3 * i + 2] - 1;
+        const double temp = (v.T[n1] + v.T[n2]) / 2.0;
+        const double smu = 110.0;
+        const double fmu = 0.017 * pow_3_2(temp / v.t_inf) * (v.t_inf + smu) / (temp + smu);
+        double rly = -(X[n2] - X[n1]);
+        double rlx = Y[n2] - Y[n1];
+        const double rmod = sqrt(rlx * rlx + rly * rly);
+        rlx = rlx / rmod;
+        rly = rly / rmod;
+        double dux = 0.0, duy = 0.0, dvx = 0.0, dvy = 0.0, press = 0.0;
 #pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-                const int nn = v.inpoel[3 * ie + jj] - 1;
-                const double nx = v.dNx[3 * ie + jj], ny = v.dNy[3 * ie + jj];
-                const double vx = v.VX[nn], vy = v.VY[nn];
-                dux = dux + nx * vx;
-                duy = duy + ny * vx;
-                dvx = dvx + nx * vy;
-                dvy = dvy + ny * vy;
-                press = press + P[nn];
-            }
-            press = press / 3.0;
-            const double txx = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dux);
-            const double txy = fmu * (duy + dvx);
-            const double tyy = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dvy);
-            fvx = fvx + (txx * rlx + txy * rly) * rmod;
-            fvy = fvy + (txy * rlx + tyy * rly) * rmod;
+        for (int jj = 0; jj < 3; ++jj) {
+            const int nn = v.inpoel[3 * ie + jj] - 1;
+            const double nx = v.dNx[3 * ie + jj], ny = v.dNy[3 * ie + jj];
+            const double vx = v.VX[nn], vy = v.VY[nn];
+            dux = dux + nx * vx;
+            duy = duy + ny * vx;
+            dvx = dvx + nx * vy;
+            dvy = dvy + ny * vy;
+            press = press + P[nn];
         }
+        press = press / 3.0;
+        const double txx = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dux);
+        const double txy = fmu * (duy + dvx);
+        const double tyy = -press - fmu * (2.0 / 3.0 * (dux + dvy) - 2.0 * dvy);
+        r[3] = (txx * rlx + txy * rly) * rmod;
+        r[4] = (txy * rlx + tyy * rly) * rmod;
     }
-    const int t = threadIdx.x;
-    red[0][t] = fx;
-    red[1][t] = fy;
-    red[2][t] = rm;
-    red[3][t] = fvx;
-    red[4][t] = fvy;
-    for (int h = kForceThreads / 2; h > 0; h >>= 1) {
-        __syncthreads();
-        if (t < h) {
+}
+
+// One wave per 64 edges: lane l forms edge 64 b + l, the wave sums the terms
+// by a fixed shuffle tree and lane 0 stores the block's five partials at
+// part[q * nb + b].
+template <bool kPress, bool kVisc>
+__global__ void __launch_bounds__(64)
+k_forces_part(int nedge, const int* __restrict__ iset, const double* __restrict__ X,
+              const double* __restrict__ Y, const double* __restrict__ P, double xref,
+              double yref, ViscIn v, double* __restrict__ part) {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int i = b * 64 + threadIdx.x;
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < nedge) force_terms<kPress, kVisc>(i, iset, X, Y, P, xref, yref, v, r);
 #pragma unroll
-            for (int q = 0; q < 5; ++q) red[q][t] = red[q][t] + red[q][t + h];
+    for (int q = 0; q < 5; ++q) {
+        if (q < 3 && !kPress) continue;
+        if (q >= 3 && !kVisc) continue;
+        const double t = wave_reduce(r[q], OpSum());
+        if (threadIdx.x == 0) part[q * nb + b] = t;
+    }
+}
+
+// One wave folds the nb block partials: lane l sums partials l, l + 64, ...
+// in that order, then the shuffle tree.  zero4 (when given) receives four zeros
+// in the same launch (the iteration counts and residuals of mesh solves that
+// stop at once on a mesh that does not move).
+template <bool kPress, bool kVisc>
+__global__ void __launch_bounds__(64)
+k_forces_fin(int nb, const double* __restrict__ part, double* __restrict__ out,
+             double* __restrict__ outv, double* __restrict__ zero4) {
+    const int l = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        if (q < 3 && !kPress) continue;
+        if (q >= 3 && !kVisc) continue;
+        double a = 0.0;
+        for (int k = l; k < nb; k += 64) a = a + part[q * nb + k];
+        a = wave_reduce(a, OpSum());
+        if (l == 0) {
+            if (q < 3)
+                out[q] = a;
+            else
+                outv[q - 3] = a;
         }
     }
-    if (t == 0) {
-        if (kPress) {
-            out[0] = red[0][0];
-            out[1] = red[1][0];
-            out[2] = red[2][0];
-        }
-        if (kVisc) {
-            outv[0] = red[3][0];
-            outv[1] = red[4][0];
-        }
-    }
+    if (zero4 && l < 4) zero4[l] = 0.0;
 }
 
 // meshMove.f90:378-389, one set
@@ -352,20 +368,30 @@ void launch_normalvel(const BcTable& bc, double* VX, double* VY, const double* W
     CFD_HIP(hipGetLastError());
 }
 
+size_t forces_part_size(int nedge) { return 5 * (size_t)std::max(1, (nedge + 63) / 64); }
+
 void launch_forces(int nedge, const int* iset, const double* X, const double* Y, const double* P,
                    double xref, double yref, double* out3, const ViscIn* visc, double* out2,
-                   hipStream_t s) {
-    if (nedge == 0 || (!out3 && !visc)) return;
+                   double* part, double* zero4, hipStream_t s) {
+    if (nedge == 0 || (!out3 && !visc)) {
+        if (zero4) {
+            k_forces_fin<false, false><<<1, 64, 0, s>>>(0, part, out3, out2, zero4);
+            CFD_HIP(hipGetLastError());
+        }
+        return;
+    }
     const ViscIn v = visc ? *visc : ViscIn{};
-    if (out3 && visc)
-        k_forces<true, true><<<1, kForceThreads, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3, v,
-                                                          out2);
-    else if (out3)
-        k_forces<true, false><<<1, kForceThreads, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3,
-                                                           v, out2);
-    else
-        k_forces<false, true><<<1, kForceThreads, 0, s>>>(nedge, iset, X, Y, P, xref, yref, out3,
-                                                           v, out2);
+    const int nb = (nedge + 63) / 64;
+    if (out3 && visc) {
+        k_forces_part<true, true><<<nb, 64, 0, s>>>(nedge, iset, X, Y, P, xref, yref, v, part);
+        k_forces_fin<true, true><<<1, 64, 0, s>>>(nb, part, out3, out2, zero4);
+    } else if (out3) {
+        k_forces_part<true, false><<<nb, 64, 0, s>>>(nedge, iset, X, Y, P, xref, yref, v, part);
+        k_forces_fin<true, false><<<1, 64, 0, s>>>(nb, part, out3, out2, zero4);
+    } else {
+        k_forces_part<false, true><<<nb, 64, 0, s>>>(nedge, iset, X, Y, P, xref, yref, v, part);
+        k_forces_fin<false, true><<<1, 64, 0, s>>>(nb, part, out3, out2, zero4);
+    }
     CFD_HIP(hipGetLastError());
 }
 
