@@ -303,6 +303,9 @@ def batch_sweep(steps: int = 100, warmup: int = 5, ncases: int = 256) -> dict:
 
 
 LARGE_RUNS = ("plain_fp64", "extrap_fp32", "learned_fp32")
+# opt-in: the plain run with the density of one node next to the body changed
+# by one ulp (the chaos floor of the force histories)
+LARGE_FLOOR = "plain_fp64_ulp"
 
 
 def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 100000,
@@ -316,18 +319,31 @@ def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 
     this size), a few steps each (the mesh solves dominate; at this size one
     plain step takes about a second, and the first solve from the start-up
     guess tens of seconds).  The three warm-up steps fill the three-solution
-    history the network reads."""
+    history the network reads.  Each run records its drag and lift
+    coefficients after every timed step; the histories of the other runs are
+    compared with the plain run's step by step (largest relative difference;
+    for lift also the largest absolute one), and so is the one-ulp floor run
+    when asked for (LARGE_FLOOR)."""
     from cfd_amd.mesh import ellipse_case
     from cfd_amd.solver import Session
     cs = ellipse_case(4096, 4096, mach=0.7, re=10000.0, motion=1, motion_amp=PITCH_AMP,
                       motion_omega=PITCH_OMEGA)
-    runs = {}
-    for name, prec, ws in (("plain_fp64", 0, 0), ("extrap_fp32", 1, 2), ("learned_fp32", 1, 1)):
+    import numpy as np
+    runs, hist = {}, {}
+    for name, prec, ws in (("plain_fp64", 0, 0), ("extrap_fp32", 1, 2), ("learned_fp32", 1, 1),
+                           (LARGE_FLOOR, 0, 0)):
         if name not in runs_wanted:
             continue
         t0 = time.perf_counter()
         S = Session(cs, precision=prec, pcg_rel=1, pcg_tol=tol, pcg_maxit=maxit, warm_start=ws,
                     gcl=1)
+        if name == LARGE_FLOOR:
+            U = S.get("U")
+            k = near_body_nodes(cs, 1)[0]
+            U[k, 0] = np.nextafter(U[k, 0], 2.0)
+            S.set("U", U)
+            S.set("U1", U)
+            del U
         t_create = time.perf_counter() - t0
         progress(f"large_ale {name}: created in {t_create:.1f} s")
         for _ in range(warmup):
@@ -337,12 +353,18 @@ def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 
                      f"{S.scalars()['PCG_ITSUM']:.0f}")
         it0 = S.scalars()["PCG_ITSUM"]
         sec = 0.0
-        for _ in range(steps):  # one step at a time: progress lines between
+        hcd, hcl = [], []
+        for j in range(steps):  # one step at a time: progress lines between
             t0 = time.perf_counter()
             S.step(1)
             S.sync()
             sec += time.perf_counter() - t0
-            progress(f"large_ale {name}: step, {sec:.1f} s so far")
+            fc = force_coefficients(S.scalars(), cs)
+            hcd.append(fc["cd"])
+            hcl.append(fc["cl"])
+            if j % 10 == 9 or j == steps - 1:
+                progress(f"large_ale {name}: {j + 1} steps, {sec:.1f} s so far, cd {fc['cd']:.6f}")
+        hist[name] = (np.array(hcd), np.array(hcl))
         sc = S.scalars()
         runs[name] = {"ms_per_step": sec / steps * 1e3, "create_s": t_create,
                       "pcg_iterations_per_step": (sc["PCG_ITSUM"] - it0) / steps,
@@ -357,6 +379,18 @@ def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 
         for k in ("extrap_fp32", "learned_fp32"):
             runs[f"{k}_cd_diff"] = _rel(runs[k]["cd"], p["cd"])
             runs[f"{k}_cl_diff"] = _rel(runs[k]["cl"], p["cl"])
+    if "plain_fp64" in hist:
+        pcd, pcl = hist["plain_fp64"]
+        for k, (cd, cl) in hist.items():
+            if k == "plain_fp64":
+                continue
+            runs[f"{k}_history"] = {
+                "steps": int(cd.size),
+                "cd_max_rel_diff": float(np.max(np.abs(cd - pcd) / np.abs(pcd))),
+                "cl_max_rel_diff": float(np.max(np.abs(cl - pcl) / np.abs(pcl))),
+                "cl_max_abs_diff": float(np.max(np.abs(cl - pcl)))}
+        runs["plain_fp64_history"] = {"cd_first_last": [float(pcd[0]), float(pcd[-1])],
+                                      "cl_first_last": [float(pcl[0]), float(pcl[-1])]}
     runs["workload"] = (f"ellipse O-grid 4096x4096 ({cs.npoin} nodes, {cs.nelem} triangles), M=0.7, "
                         f"Re=10000, alpha(t) = 5 deg sin(0.2 t), GCL on, relative PCG rule {tol:g} "
                         f"(at most {maxit} iterations), {warmup} untimed + {steps} timed steps")
@@ -503,7 +537,10 @@ def main():
     ap.add_argument("--no-large-ale", action="store_true",
                     help="skip the 4096 x 4096 pitching ellipse (about two minutes)")
     ap.add_argument("--large-ale-runs", default=",".join(LARGE_RUNS),
-                    help="comma-separated runs of the large ALE section (profiling)")
+                    help="comma-separated runs of the large ALE section: "
+                         f"{', '.join(LARGE_RUNS)} and, opt-in, {LARGE_FLOOR} (the plain run "
+                         "with one node's density changed by one ulp: the force histories' "
+                         "chaos floor)")
     ap.add_argument("--large-ale-steps", type=int, default=2)
     ap.add_argument("--large-ale-warmup", type=int, default=3)
     ap.add_argument("--only", choices=["large_ale", "ale"], default=None,
