@@ -141,9 +141,11 @@ int cfd_rk(cfd_mesh *m, const cfd_consts *c, const cfd_bc_lists *bc, int nrk, do
 /* FORCE_VISC (ns2DComp.ALE.f90:819-893) for one force set: the traction of
  * the full stress tensor (element-average pressure and the set element's P1
  * velocity gradients, Sutherland viscosity at the edge-average temperature)
- * on each edge (n1, n2, elem) of iset(3,nedge), times its length, summed in
- * edge order into *f_vx, *f_vy.  The per-edge skin friction the reference
- * writes to SKIN.DAT is not produced. */
+ * on each edge (n1, n2, elem) of iset(3,nedge), times its length, summed
+ * into *f_vx, *f_vy in a fixed order (64-edge partial sums by a fixed tree,
+ * then in order; the reference sums in edge order, so the two agree at
+ * round-off).  The per-edge skin friction the reference writes to SKIN.DAT
+ * is not produced. */
 int cfd_force_visc(cfd_mesh *m, int nedge, const int *iset, const double *X, const double *Y,
                    const double *P, const double *T, const double *VEL_X, const double *VEL_Y,
                    const double *dNx, const double *dNy, double t_inf, double *f_vx,

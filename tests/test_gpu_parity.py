@@ -206,7 +206,10 @@ def test_fuente(cyl, mctx):
 
 def test_force_visc_entry(cyl, mctx):
     """cfd_force_visc (FORCE_VISC, ns2DComp.ALE.f90:819-893) against the
-    oracle on a developed state; the sums run in the same edge order."""
+    oracle on a developed state (the device sums the edge terms by a fixed
+    tree, the oracle in edge order: equal within round-off of the terms'
+    magnitudes); the set repeated 101 times (more than 64 partial sums to
+    fold) gives 101 times the force."""
     cs, o = cyl
     s = o.s
     iset = np.asarray(cs.lists["iset"], np.int32)
@@ -217,6 +220,12 @@ def test_force_visc_entry(cyl, mctx):
     print("FORCE_VISC got", got, "ref", ref, "scale", sc)
     for a, b in zip(got, ref):
         assert abs(a - b) <= 1e-14 * sc
+    rep = np.tile(iset.reshape(-1, 3), (101, 1))
+    assert rep.shape[0] > 64 * 64
+    got_r = mctx.force_visc(rep, s["X"], s["Y"], s["P"], s["T"], s["VEL_X"], s["VEL_Y"],
+                            s["DNX"], s["DNY"], cs.t_inf)
+    for a, b in zip(got_r, ref):
+        assert abs(a - 101 * b) <= 101 * 1e-14 * sc
     # an empty set gives zero, a bad element id is refused
     assert mctx.force_visc(np.zeros((0, 3), np.int32), s["X"], s["Y"], s["P"], s["T"],
                            s["VEL_X"], s["VEL_Y"], s["DNX"], s["DNY"], cs.t_inf) == (0.0, 0.0)
