@@ -308,7 +308,7 @@ LARGE_RUNS = ("plain_fp64", "extrap_fp32", "learned_fp32")
 LARGE_FLOOR = "plain_fp64_ulp"
 
 
-def large_ale(steps: int = 2, warmup: int = 3, tol: float = 1e-10, maxit: int = 100000,
+def large_ale(steps: int = 10, warmup: int = 3, tol: float = 1e-10, maxit: int = 100000,
               runs_wanted=LARGE_RUNS) -> dict:
     """The large ALE run: pitching ellipse O-grid 4096 x 4096 (16,777,216 nodes,
     33,546,240 triangles), M=0.7, Re=10000, alpha(t) = 5 deg sin(0.2 t), the
@@ -541,7 +541,7 @@ def main():
                          f"{', '.join(LARGE_RUNS)} and, opt-in, {LARGE_FLOOR} (the plain run "
                          "with one node's density changed by one ulp: the force histories' "
                          "chaos floor)")
-    ap.add_argument("--large-ale-steps", type=int, default=2)
+    ap.add_argument("--large-ale-steps", type=int, default=10)
     ap.add_argument("--large-ale-warmup", type=int, default=3)
     ap.add_argument("--only", choices=["large_ale", "ale"], default=None,
                     help="run only this section and print its JSON (profiling)")
