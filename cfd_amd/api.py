@@ -169,6 +169,28 @@ class MeshContext:
         check(lib().cfd_normalvel(self._h, dptr(vx), dptr(vy), dptr(_f64(W_X)), dptr(_f64(W_Y))))
         return vx, vy
 
+    def smoothing(self, X, Y, fixed, dX=None, dY=None, order: int = 0):
+        """smoothing.f90:21-75 (smoothing_mod::smoothing) on copies of X, Y:
+        returns (X, Y, sweeps, H_MIN_GLOBAL).  fixed: nodes held (nonzero).
+        dX, dY: the reference's optional arguments (setTol_move after the
+        mesh check; the mode then stays on for this context).  order 0: the
+        reference's node order (bit for bit, at most 65536 nodes); 1: colour
+        by colour, in parallel."""
+        x, y = _f64(X).copy(), _f64(Y).copy()
+        fx = _i32(np.asarray(fixed) != 0)
+        dx = None if dX is None else _f64(dX)
+        dy = None if dY is None else _f64(dY)
+        n, h = C.c_int(), C.c_double()
+        check(lib().cfd_smoothing(self._h, dptr(x), dptr(y), iptr(fx),
+                                  None if dx is None else dptr(dx),
+                                  None if dy is None else dptr(dy), int(order), C.byref(n),
+                                  C.byref(h)))
+        return x, y, n.value, h.value
+
+    def set_tol_move(self, dX, dY):
+        """smoothing.f90:386-401 setTol_move: tol_move = |dX| + |dY|"""
+        check(lib().cfd_smoothing_set_tol_move(self._h, dptr(_f64(dX)), dptr(_f64(dY))))
+
     def force_visc(self, iset, X, Y, P, T, VEL_X, VEL_Y, dNx, dNy, t_inf):
         """ns2DComp.ALE.f90:819-893 for one set of edges (n1, n2, elem):
         (F_VX, F_VY)"""

@@ -18,7 +18,8 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libcfd_amd.so")
 SOURCES = ["tables.hip", "geom.hip", "flow.hip", "pcg.hip", "patches.hip", "mlp.hip",
-           "warmstart.hip", "state.hip", "batch.hip", "api.hip", "restart.hip"]
+           "warmstart.hip", "state.hip", "batch.hip", "api.hip", "restart.hip",
+           "smooth.hip"]
 HEADERS = ["cfd_internal.hpp", "reduce.hpp", "crmath.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

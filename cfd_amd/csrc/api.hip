@@ -63,6 +63,11 @@ struct cfd_mesh {
     DBuf<double> w_x_old, w_y_old, area_old;
     bool has_w = false, has_area = false;
     PcgWork pcg;
+    // smoothing_mod saved state: tol_move and SWITCH_TOL_MOVE stay set once
+    // setTol_move has run (smoothing.f90:15-17)
+    SmoothPlan smooth;
+    DBuf<double> tol_move;
+    bool switch_tol_move = false;
 
     double* scratch(const char* name, size_t n) {
         auto& b = dscratch[name];
@@ -338,6 +343,51 @@ int cfd_normalvel(cfd_mesh* m, double* VEL_X, double* VEL_Y, const double* W_X, 
         m->get(VEL_X, (const double*)vx, np);
         m->get(VEL_Y, (const double*)vy, np);
         m->sync();
+    })
+}
+
+// smoothing_mod::setTol_move (smoothing.f90:386-401)
+static void set_tol_move(cfd_mesh* m, const double* dX, const double* dY) {
+    const int np = m->t.npoin;
+    std::vector<double> tm(np);
+    for (int i = 0; i < np; ++i) tm[i] = std::fabs(dX[i]) + std::fabs(dY[i]);
+    if (m->tol_move.n < (size_t)np) m->tol_move.alloc(np);
+    m->tol_move.upload(tm.data(), np, m->s);
+    m->switch_tol_move = true;
+}
+
+int cfd_smoothing_set_tol_move(cfd_mesh* m, const double* dX, const double* dY) {
+    CFD_TRY({
+        CFD_ARG(m && dX && dY, "null argument");
+        set_tol_move(m, dX, dY);
+        m->sync();
+    })
+}
+
+int cfd_smoothing(cfd_mesh* m, double* X, double* Y, const int* fixed, const double* dX,
+                  const double* dY, int order, int* sweeps, double* hmin) {
+    CFD_TRY({
+        CFD_ARG(m && X && Y && fixed, "null argument");
+        CFD_ARG(!dX == !dY, "dX and dY go together");
+        CFD_ARG(order == 0 || order == 1, "order must be 0 (node order) or 1 (by colour)");
+        const int np = m->t.npoin;
+        CFD_ARG(order == 1 || np <= kSmoothSeqMaxNodes,
+                "order 0 walks the nodes on one lane: at most 65536 nodes (use order 1)");
+        m->smooth.build(m->t, fixed, m->s);
+        CFD_ARG(m->smooth.max_elems <= 20,
+                "a node has more than 20 elements (the reference's MAX_ELEM_PER_NODE)");
+        double* dXs = m->put("smX", X, np);
+        double* dYs = m->put("smY", Y, np);
+        int n = 0;
+        if (smooth_check(m->t, m->smooth, dXs, dYs, hmin, m->s)) {
+            if (dX) set_tol_move(m, dX, dY);  // after checkMesh (smoothing.f90:44-46)
+            n = smooth_run(m->t, m->smooth, dXs, dYs, m->switch_tol_move ? m->tol_move.p : nullptr,
+                           order, m->s);
+            m->get(X, (const double*)dXs, np);
+            m->get(Y, (const double*)dYs, np);
+            m->sync();
+        }
+        if (sweeps) *sweeps = n;
     })
 }
 

@@ -434,6 +434,27 @@ void launch_normales(const BcTable& bc, int nwallnodes, const int* wnode_bc, con
                      const int* wnode_edges, const int* wall, const double* X, const double* Y,
                      hipStream_t s);
 
+// Start-up mesh smoothing (smooth.hip, smoothing.f90): the colouring and
+// work space of one mesh for a given set of fixed nodes
+constexpr int kSmoothSeqMaxNodes = 65536;  // order 0 (one lane) above this: refused
+struct SmoothPlan {
+    int max_elems = 0;              // most elements around one node
+    double hmin = 1.0;              // H_MIN_GLOBAL of the last check
+    std::vector<int> colour_ptr;    // free nodes of colour c: colour_nodes[ptr[c], ptr[c+1])
+    DBuf<int> colour_nodes, fixed, smoothable;
+    DBuf<double> dmove, partials, scal;
+    void build(const Tables& t, const int* fixed_host, hipStream_t s);
+};
+// checkMesh: the smoothable marks and H_MIN_GLOBAL (*hmin_host); false when
+// no element is below the quality bound (the reference then returns)
+bool smooth_check(const Tables& t, SmoothPlan& plan, double* X, double* Y, double* hmin_host,
+                  hipStream_t s);
+// the Laplacian passes and the sweeps after smooth_check: X, Y (device) in
+// place; tol_move (device, may be null); order 0 = the reference's node
+// order, 1 = colour by colour.  Returns the optimisation sweeps run.
+int smooth_run(const Tables& t, SmoothPlan& plan, double* X, double* Y, const double* tol_move,
+               int order, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // Session state helpers (state.hip), shared by the session and the batch
 // ---------------------------------------------------------------------------

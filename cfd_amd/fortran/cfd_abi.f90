@@ -218,6 +218,25 @@ module cfd_abi
             real(c_double), value :: t_inf
             real(c_double), intent(out) :: f_vx, f_vy
         end function
+        ! smoothing_mod::smoothing (smoothing.f90:21-75); dx, dy: c_null_ptr or
+        ! the addresses of the optional arguments
+        integer(c_int) function cfd_smoothing(m, x, y, fixed, dx, dy, order, sweeps, hmin) &
+                bind(c, name='cfd_smoothing')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m, dx, dy
+            real(c_double), intent(inout) :: x(*), y(*)
+            integer(c_int), intent(in) :: fixed(*)
+            integer(c_int), value :: order
+            integer(c_int), intent(out) :: sweeps
+            real(c_double), intent(out) :: hmin
+        end function
+        ! smoothing_mod::setTol_move (smoothing.f90:386-401)
+        integer(c_int) function cfd_smoothing_set_tol_move(m, dx, dy) &
+                bind(c, name='cfd_smoothing_set_tol_move')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: m
+            real(c_double), intent(in) :: dx(*), dy(*)
+        end function
         ! RK (subrutinas.f90:645-849, NGAS = 0)
         integer(c_int) function cfd_rk(m, c, bc, nrk, dtmin, dtl, nodes, elems) &
                 bind(c, name='cfd_rk')

@@ -6,7 +6,8 @@
 !   BiconjGrad), mLaplace.f90 (module Mlaplace), calcRHS.f90 (module
 !   calcRHS_mod), gcl.f90 (module gcl_mod), and from subrutinas.f90 the
 !   external subroutines deriv, MASAS, deltat, ESTAB, FUENTE and RK plus
-!   module Mnormales (normales, normalvel).
+!   module Mnormales (normales, normalvel), and smoothing.f90 (module
+!   smoothing_mod), which flang cannot compile as the reference wrote it.
 ! The hidden inputs the reference reads by USE (InputData, MeshData,
 ! Mvariables, MVELOCIDADES, MVARIABGEN, MESTABILIZACION) are read the same
 ! way here, so this file compiles only together with the reference's own
@@ -209,6 +210,50 @@ contains
         if (with_normals) call cfd_check(cfd_normales(shim_mesh_data(), bc, X, Y), 'normales')
     end subroutine shim_bc_call
 end module Mnormales
+
+! smoothing.f90:1-387: the start-up mesh smoothing.  smoothing_order picks the
+! node order: 0 the reference's, bit for bit its result (meshes of at most
+! 65536 nodes); 1 colour by colour, in parallel (the same per-node updates in
+! another order); -1 (the default) 0 up to 65536 nodes, else 1.  The
+! tol_move mode that setTol_move switches on stays on in the mesh context, as
+! in the reference's module.
+module smoothing_mod
+    use cfd_shim_ctx
+    implicit none
+    private
+    integer, public, save :: smoothing_order = -1
+    public :: smoothing, setTol_move
+contains
+    subroutine smoothing(X, Y, inpoel, fixed, npoin0, nelem0, dX, dY)
+        integer, intent(in) :: npoin0, nelem0, inpoel(3, nelem0)
+        real*8, intent(inout) :: X(npoin0), Y(npoin0)
+        logical, intent(in) :: fixed(npoin0)
+        real*8, intent(in), optional :: dX(npoin0), dY(npoin0)
+        integer(c_int), allocatable :: ifixed(:)
+        real(c_double), allocatable, target :: tdx(:), tdy(:)
+        type(c_ptr) :: pdx, pdy
+        integer(c_int) :: sweeps, order
+        real(c_double) :: hmin
+        ifixed = merge(1, 0, fixed)
+        pdx = c_null_ptr
+        pdy = c_null_ptr
+        if (present(dX) .and. present(dY)) then
+            tdx = dX
+            tdy = dY
+            pdx = c_loc(tdx)
+            pdy = c_loc(tdy)
+        end if
+        order = smoothing_order
+        if (order < 0) order = merge(0, 1, npoin0 <= 65536)
+        call cfd_check(cfd_smoothing(shim_mesh(inpoel, nelem0, npoin0), X, Y, ifixed, pdx, pdy, &
+                                     order, sweeps, hmin), 'smoothing')
+    end subroutine smoothing
+
+    subroutine setTol_move(dX, dY)
+        real*8, dimension(:), intent(in) :: dX, dY
+        call cfd_check(cfd_smoothing_set_tol_move(shim_mesh_data(), dX, dY), 'setTol_move')
+    end subroutine setTol_move
+end module smoothing_mod
 
 ! RK's call: the boundary lists are copied into TARGET arrays and the module
 ! arrays are passed through TARGET dummies, so their addresses can be handed

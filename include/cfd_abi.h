@@ -151,6 +151,27 @@ int cfd_force_visc(cfd_mesh *m, int nedge, const int *iset, const double *X, con
                    const double *dNx, const double *dNy, double t_inf, double *f_vx,
                    double *f_vy);
 
+/* smoothing_mod::smoothing (smoothing.f90:21-75): start-up mesh smoothing of
+ * X, Y (host, in place) with the nodes where fixed[i] != 0 held (the main
+ * program fixes the moving and fixed-move lists, ns2DComp.ALE.f90:62-76).
+ * Nodes of elements with quality below 0.85 become smoothable; two
+ * Laplacian passes, then up to 100 sweeps of quality-gradient moves.  dX, dY
+ * (both or neither; they are the reference's optional arguments) call
+ * setTol_move after the mesh check, as the reference does; the tol_move mode
+ * then stays on for this mesh context (module state in the reference).
+ * order 0: the reference's node order, bit for bit its result (one lane
+ * walks the nodes: at most 65536 nodes); order 1: colour by colour over a
+ * greedy node colouring, in parallel (the same per-node updates in another
+ * order: a different smoothing of the same kind).  *sweeps (may be null):
+ * the optimisation sweeps run, 0 when nothing was smoothable; *hmin (may be
+ * null): H_MIN_GLOBAL.  Refused when a node has more than 20 elements (the
+ * reference's MAX_ELEM_PER_NODE). */
+int cfd_smoothing(cfd_mesh *m, double *X, double *Y, const int *fixed, const double *dX,
+                  const double *dY, int order, int *sweeps, double *hmin);
+/* smoothing_mod::setTol_move (smoothing.f90:386-401): tol_move = |dX| + |dY|
+ * per node; the tol_move mode stays on for this mesh context */
+int cfd_smoothing_set_tol_move(cfd_mesh *m, const double *dX, const double *dY);
+
 /* BiconjGrad::biCG (biconjGrad.f90:8-62): Jacobi-PCG on a CSR matrix with
  * Dirichlet nodes; x is the initial guess in, the solution out.  Extensions
  * the reference lacks: *iters (iterations run), *rz (final |r.z|). */

@@ -8,6 +8,7 @@
  */
 #include "cfd_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <omp.h>
 #include <stdlib.h>
@@ -734,4 +735,234 @@ void orc_gcl(int nelem, int npoin, const int *inpoel, double *M, const double *W
         M[p] = M[p] + dt * (tot1[p] + tot2[p]) / 2.0;
     free(tot1);
     free(tot2);
+}
+
+/* ------------------------------------------------------------------ */
+/* smoothing.f90 smoothing_mod: start-up mesh smoothing, sequential      */
+/* ------------------------------------------------------------------ */
+#define SM_MAXE 20 /* MAX_ELEM_PER_NODE, smoothing.f90:6 */
+
+typedef struct {
+    int npoin;
+    const int *inpoel, *esup1, *esup2, *psup1, *psup2;
+    double *X, *Y;
+    int *smoothable;
+    const double *tol_move; /* NULL: SWITCH_TOL_MOVE false */
+    double hmin_global;
+} sm_ctx;
+
+/* smoothing.f90:357-365 h */
+static double sm_h(double X1, double X2, double X3, double Y1, double Y2, double Y3)
+{
+    double area = X2 * Y3 + X3 * Y1 + X1 * Y2 - (X2 * Y1 + X3 * Y2 + X1 * Y3);
+    double d1 = fabs(X3 - X2) + fabs(Y3 - Y2);
+    double d2 = fabs(X1 - X3) + fabs(Y1 - Y3);
+    double d3 = fabs(X2 - X1) + fabs(Y2 - Y1);
+    double d = d1 + d2 + d3;
+    return fabs(area) / d;
+}
+
+static double sm_mu_elem(const sm_ctx *c, int e)
+{
+    const int *n = c->inpoel + 3 * e;
+    return quality_mu(c->X[n[0] - 1], c->X[n[1] - 1], c->X[n[2] - 1], c->Y[n[0] - 1],
+                      c->Y[n[1] - 1], c->Y[n[2] - 1]);
+}
+
+/* smoothing.f90:174-197 getMu_vec; returns the count, min_idx 1-based */
+static int sm_mu_vec(const sm_ctx *c, int ip, double *mu_vec, int *min_idx)
+{
+    int cnt = 0;
+    double mu_min = 1;
+    if (min_idx) *min_idx = 1;
+    for (int s = c->esup2[ip - 1] + 1; s <= c->esup2[ip]; ++s) {
+        double mu1 = sm_mu_elem(c, c->esup1[s - 1] - 1);
+        int k = s - c->esup2[ip - 1];
+        mu_vec[k - 1] = mu1;
+        cnt = k;
+        if (min_idx && mu1 < mu_min) {
+            mu_min = mu1;
+            *min_idx = k;
+        }
+    }
+    return cnt;
+}
+
+/* smoothing.f90:199-215 getMu_min */
+static double sm_mu_min(const sm_ctx *c, int ip)
+{
+    double m = 1;
+    for (int s = c->esup2[ip - 1] + 1; s <= c->esup2[ip]; ++s) {
+        double mu1 = sm_mu_elem(c, c->esup1[s - 1] - 1);
+        if (mu1 <= m) m = mu1;
+    }
+    return m;
+}
+
+/* smoothing.f90:272-287 getH_min */
+static double sm_h_min(const sm_ctx *c, int ip)
+{
+    double m = 1;
+    for (int s = c->esup2[ip - 1] + 1; s <= c->esup2[ip]; ++s) {
+        const int *n = c->inpoel + 3 * (c->esup1[s - 1] - 1);
+        double h1 = sm_h(c->X[n[0] - 1], c->X[n[1] - 1], c->X[n[2] - 1], c->Y[n[0] - 1],
+                         c->Y[n[1] - 1], c->Y[n[2] - 1]);
+        if (h1 < m) m = h1;
+    }
+    return m;
+}
+
+/* smoothing.f90:77-111 laplacianSmoothing, one pass in node order */
+static void sm_laplacian(sm_ctx *c, const int *fixed)
+{
+    for (int ip = 1; ip <= c->npoin; ++ip) {
+        if (!(c->smoothable[ip - 1] && !fixed[ip - 1])) continue;
+        double xn = 0, yn = 0;
+        double mu_old = sm_mu_min(c, ip);
+        double xo = c->X[ip - 1], yo = c->Y[ip - 1];
+        int n = c->psup2[ip] - c->psup2[ip - 1];
+        for (int s = c->psup2[ip - 1] + 1; s <= c->psup2[ip]; ++s) {
+            xn = xn + c->X[c->psup1[s - 1] - 1];
+            yn = yn + c->Y[c->psup1[s - 1] - 1];
+        }
+        xn = xn / n;
+        yn = yn / n;
+        if (c->tol_move) {
+            double dX = xn - c->X[ip - 1], dY = yn - c->Y[ip - 1];
+            double tm = c->tol_move[ip - 1];
+            c->X[ip - 1] = c->X[ip - 1] + dX / (fabs(dX) + fabs(dY)) * tm / 2 * 3.0;
+            c->Y[ip - 1] = c->Y[ip - 1] + dY / (fabs(dX) + fabs(dY)) * tm / 2 * 3.0;
+        } else {
+            c->X[ip - 1] = xn;
+            c->Y[ip - 1] = yn;
+        }
+        if (sm_mu_min(c, ip) < mu_old) { /* if worse, restore */
+            c->X[ip - 1] = xo;
+            c->Y[ip - 1] = yo;
+        }
+    }
+}
+
+/* smoothing.f90:217-236 getG: one-sided differences of every element's mu */
+static void sm_grad(sm_ctx *c, int ip, const double *mu_vec, double *gx, double *gy)
+{
+    double delta = c->hmin_global * 1e-2;
+    double xo = c->X[ip - 1];
+    c->X[ip - 1] = c->X[ip - 1] + delta;
+    int cnt = sm_mu_vec(c, ip, gx, NULL);
+    for (int k = 0; k < cnt; ++k) gx[k] = (gx[k] - mu_vec[k]) / delta;
+    c->X[ip - 1] = xo;
+    double yo = c->Y[ip - 1];
+    c->Y[ip - 1] = c->Y[ip - 1] + delta;
+    sm_mu_vec(c, ip, gy, NULL);
+    for (int k = 0; k < cnt; ++k) gy[k] = (gy[k] - mu_vec[k]) / delta;
+    c->Y[ip - 1] = yo;
+}
+
+/* smoothing.f90:238-270 getStep */
+static double sm_step(const sm_ctx *c, int ip, const double *gx, const double *gy,
+                      const double *mu_vec, int min_idx, int cnt)
+{
+    double gxm = gx[min_idx - 1], gym = gy[min_idx - 1], mum = mu_vec[min_idx - 1];
+    double g2 = gxm * gxm + gym * gym;
+    double step;
+    if (c->tol_move)
+        step = c->tol_move[ip - 1] / ((fabs(gxm) + fabs(gym)) * 100.0);
+    else
+        step = sm_h_min(c, ip) * 3.0 / (fabs(gxm) + fabs(gym));
+    for (int i = 0; i < cnt; ++i) {
+        double gg = gxm * gx[i] + gym * gy[i];
+        if (gg < 0) {
+            double step1 = (mu_vec[i] - mum) / (g2 - gg);
+            if (step1 < step) step = step1;
+        }
+    }
+    return step;
+}
+
+/* smoothing.f90:113-171 moveIpoin; returns d_move */
+static double sm_move(sm_ctx *c, int ip, double *mu_vec, int min_idx)
+{
+    double gx[SM_MAXE], gy[SM_MAXE];
+    double x0 = c->X[ip - 1], y0 = c->Y[ip - 1];
+    int cnt = c->esup2[ip] - c->esup2[ip - 1];
+    for (int it = 0; it < 2; ++it) { /* MITER */
+        double mu_min = mu_vec[min_idx - 1];
+        double xo = c->X[ip - 1], yo = c->Y[ip - 1];
+        sm_grad(c, ip, mu_vec, gx, gy);
+        double step = sm_step(c, ip, gx, gy, mu_vec, min_idx, cnt);
+        int accepted = 0;
+        for (int j = 0; j < 8; ++j) { /* NTRY: the tries accumulate */
+            double dx = step * gx[min_idx - 1], dy = step * gy[min_idx - 1];
+            c->X[ip - 1] = c->X[ip - 1] + dx;
+            c->Y[ip - 1] = c->Y[ip - 1] + dy;
+            int mi;
+            sm_mu_vec(c, ip, mu_vec, &mi);
+            if (mu_vec[mi - 1] > mu_min * 1.0) {
+                min_idx = mi;
+                accepted = 1;
+                break;
+            }
+            step = .5 * step;
+        }
+        if (!accepted) {
+            c->X[ip - 1] = xo;
+            c->Y[ip - 1] = yo;
+            break;
+        }
+    }
+    double dxm = c->X[ip - 1] - x0, dym = c->Y[ip - 1] - y0;
+    double d_move = dxm * dxm + dym * dym;
+    if (d_move > DBL_MIN) /* update_list, smoothing.f90:289-298 */
+        for (int s = c->psup2[ip - 1] + 1; s <= c->psup2[ip]; ++s)
+            c->smoothable[c->psup1[s - 1] - 1] = 1;
+    return d_move;
+}
+
+int orc_smoothing(int nelem, int npoin, const int *inpoel, const int *esup1, const int *esup2,
+                  const int *psup1, const int *psup2, double *X, double *Y, const int *fixed,
+                  const double *tol_move, double *hmin_out, int max_sweeps)
+{
+    sm_ctx c = {npoin, inpoel, esup1, esup2, psup1, psup2, X, Y, NULL, tol_move, 1.0};
+    for (int ip = 1; ip <= npoin; ++ip)
+        if (esup2[ip] - esup2[ip - 1] > SM_MAXE) return -1;
+    int *sm = (int *)calloc((size_t)npoin, sizeof(int));
+    c.smoothable = sm;
+    /* checkMesh, smoothing.f90:367-385 */
+    for (int e = 0; e < nelem; ++e) {
+        const int *n = inpoel + 3 * e;
+        if (sm_mu_elem(&c, e) < .85)
+            for (int k = 0; k < 3; ++k) sm[n[k] - 1] = 1;
+        double h1 = sm_h(X[n[0] - 1], X[n[1] - 1], X[n[2] - 1], Y[n[0] - 1], Y[n[1] - 1],
+                         Y[n[2] - 1]);
+        if (h1 < c.hmin_global) c.hmin_global = h1;
+    }
+    if (hmin_out) *hmin_out = c.hmin_global;
+    int any = 0;
+    for (int ip = 0; ip < npoin; ++ip) any |= sm[ip];
+    if (!any) {
+        free(sm);
+        return 0;
+    }
+    /* smoothing.f90:46-65 */
+    for (int it = 0; it < 2; ++it) sm_laplacian(&c, fixed);
+    double tol_dist = 1e-2 * c.hmin_global;
+    int iter;
+    double mu_vec[SM_MAXE];
+    for (iter = 1; iter <= max_sweeps; ++iter) { /* NITER = 100 */
+        double d_max = 0;
+        for (int ip = 1; ip <= npoin; ++ip) {
+            if (!(sm[ip - 1] && !fixed[ip - 1])) continue;
+            sm[ip - 1] = 0;
+            int mi;
+            sm_mu_vec(&c, ip, mu_vec, &mi);
+            if (mu_vec[mi - 1] < .85) {
+                double d = sm_move(&c, ip, mu_vec, mi);
+                if (d > d_max) d_max = d;
+            }
+        }
+        if (d_max < tol_dist) break;
+    }
+    free(sm);
+    return iter > max_sweeps ? max_sweeps : iter;
 }

@@ -114,6 +114,16 @@ void orc_gcl(int nelem, int npoin, const int *inpoel, double *M, const double *W
              const double *dNx, const double *dNy, const double *area,
              const double *area_old, double dt, int mode);
 
+/* smoothing.f90 smoothing (start-up mesh smoothing) in the reference's node
+ * order: X, Y in place; fixed(npoin) nonzero = fixed; tol_move(npoin) or
+ * NULL (setTol_move not called).  Returns the optimisation sweeps run (0:
+ * nothing smoothable), -1 when a node has more than 20 elements (the
+ * reference's MAX_ELEM_PER_NODE); *hmin_out = H_MIN_GLOBAL.  max_sweeps: the
+ * reference's NITER = 100 (fewer only to look at intermediate states). */
+int orc_smoothing(int nelem, int npoin, const int *inpoel, const int *esup1, const int *esup2,
+                  const int *psup1, const int *psup2, double *X, double *Y, const int *fixed,
+                  const double *tol_move, double *hmin_out, int max_sweeps);
+
 #ifdef __cplusplus
 }
 #endif

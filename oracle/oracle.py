@@ -90,6 +90,9 @@ def lib():
         L.orc_get_threads.restype = _i
         L.orc_set_threads(1)
         L.orc_gcl.argtypes = [_i, _i, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _d, _i]
+        L.orc_smoothing.argtypes = [_i, _i, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _ip, C.c_void_p,
+                                    _pd, _i]
+        L.orc_smoothing.restype = _i
         _lib = L
     return _lib
 
@@ -133,6 +136,26 @@ def lap_structure(npoin, psup1, psup2):
     idx = np.zeros(psup1.size + npoin, np.int32)
     lib().orc_lap_structure(npoin, psup1, psup2, rowptr, idx)
     return rowptr, idx
+
+
+def smoothing(inpoel, npoin, X, Y, fixed, tol_move=None, tabs=None, max_sweeps=100):
+    """smoothing.f90 smoothing in the reference's node order on copies of X,
+    Y: (X, Y, sweeps, H_MIN_GLOBAL).  tol_move: |dX| + |dY| per node when
+    setTol_move has been called (smoothing.f90:386-401), else None.
+    max_sweeps: the reference's 100 (fewer to look at intermediate states)."""
+    ip = np.ascontiguousarray(inpoel, np.int32)
+    t = tabs or tables(ip, npoin)
+    Xs, Ys = np.array(X, np.float64), np.array(Y, np.float64)
+    fx = np.ascontiguousarray(np.asarray(fixed) != 0, np.int32)
+    tm = None if tol_move is None else np.ascontiguousarray(tol_move, np.float64)
+    h = C.c_double()
+    n = lib().orc_smoothing(ip.shape[0], npoin, ip, t["esup1"], t["esup2"], t["psup1"],
+                            t["psup2"], Xs, Ys, fx,
+                            None if tm is None else tm.ctypes.data_as(C.c_void_p), C.byref(h),
+                            int(max_sweeps))
+    if n < 0:
+        raise ValueError("a node has more than 20 elements (MAX_ELEM_PER_NODE)")
+    return Xs, Ys, n, h.value
 
 
 def tables(inpoel, npoin):

@@ -179,6 +179,38 @@ def test_fortran_session_driver(tmp_path, precision):
 
 
 @pytest.mark.ref
+@pytest.mark.ref
+def test_reference_main_program_links_against_the_shims():
+    """The reference's main program (ns2DComp.ALE.f90, unchanged) compiles
+    against the shim modules and links with them, the reference's module
+    sources, its loader and meshMove.f90, and libcfd_amd.so, with no symbol
+    left unresolved: every hot-path routine it calls (getEsup/getPsup through
+    initialize, deriv, MASAS, laplace, NORMALES, deltat, RK, biCG through
+    fluidStructure) and smoothing_mod (start-up smoothing; the reference's
+    smoothing.f90 does not compile with flang) come from the shims."""
+    if not os.path.isdir("/root/reference") or shutil.which("flang") is None:
+        pytest.skip("needs the reference sources and flang")
+    r = subprocess.run(["make", "-s", "-C", ROOT, "-f", "oracle/Makefile.ref", "mainprog"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    exe = os.path.join(ROOT, "oracle", "_ref", "ns2d_shims")
+    main_o = os.path.join(ROOT, "oracle", "_ref", "mainp", "ns2DComp.ALE.o")
+    undefined = subprocess.run(["nm", "--undefined-only", main_o], capture_output=True, text=True,
+                               check=True).stdout
+    shim = _defined(SHIM_OBJ)
+    # what the main program takes from the shims
+    for sym in ("_QMsmoothing_modPsmoothing", "_QMmlaplacePlaplace", "_QMmnormalesPnormales",
+                "deltat_", "deriv_", "masas_", "rk_"):
+        assert sym in undefined, sym
+        assert sym in shim, sym
+    assert "_QMsmoothing_modPsettol_move" in shim
+    # the executable resolves everything (the link has no unresolved-symbol allowance)
+    assert os.path.isfile(exe)
+    dyn = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True,
+                         check=True).stdout
+    assert "cfd_rk" in dyn and "cfd_smoothing" in dyn
+
+
 def test_kept_subrutinas_compile_against_the_shims(tmp_path):
     """The routines of subrutinas.f90 that a drop-in keeps (CUARTO_ORDEN,
     fixvel, FIX, ADAMSB: everything the shims do not replace) compile
