@@ -212,7 +212,7 @@ contains
 end module Mnormales
 
 ! smoothing.f90:1-387: the start-up mesh smoothing.  smoothing_order picks the
-! node order: 0 the reference's, bit for bit its result (meshes of at most
+! node order: 0 the reference's (one lane walks the nodes; meshes of at most
 ! 65536 nodes); 1 colour by colour, in parallel (the same per-node updates in
 ! another order); -1 (the default) 0 up to 65536 nodes, else 1.  The
 ! tol_move mode that setTol_move switches on stays on in the mesh context, as

@@ -159,8 +159,9 @@ int cfd_force_visc(cfd_mesh *m, int nedge, const int *iset, const double *X, con
  * (both or neither; they are the reference's optional arguments) call
  * setTol_move after the mesh check, as the reference does; the tol_move mode
  * then stays on for this mesh context (module state in the reference).
- * order 0: the reference's node order, bit for bit its result (one lane
- * walks the nodes: at most 65536 nodes); order 1: colour by colour over a
+ * order 0: the reference's node order (one lane walks the nodes: at most
+ * 65536 nodes; bit for bit the oracle's sequential restatement of
+ * smoothing.f90, which flang cannot compile); order 1: colour by colour over a
  * greedy node colouring, in parallel (the same per-node updates in another
  * order: a different smoothing of the same kind).  *sweeps (may be null):
  * the optimisation sweeps run, 0 when nothing was smoothable; *hmin (may be
