@@ -417,11 +417,11 @@ def read_traffic(name: str):
 DEVELOP_FIELDS = ("U", "T", "VEL_X", "VEL_Y", "RHO", "E", "P", "RMACH")
 
 
-def config2_case(nr: int = 1024, nt: int = 2048):
-    """Config 2's cylinder: M 0.8, Re 1000, fixed mesh, at an acoustic CFL of
-    0.5 (FSAFE = 0.5 M / (1 + M); deltat bounds only the advective and
-    viscous rates, and at FSAFE 0.5 the M 0.8 run develops negative
-    temperatures within a few hundred steps)."""
+def cylinder_bench_case(nr: int = 1024, nt: int = 2048):
+    """The bench's cylinder O-grid (1024 x 2048 by default): M 0.8, Re 1000,
+    fixed mesh, at an acoustic CFL of 0.5 (FSAFE = 0.5 M / (1 + M); deltat
+    bounds only the advective and viscous rates, and at FSAFE 0.5 the M 0.8
+    run develops negative temperatures within a few hundred steps)."""
     from cfd_amd.mesh import acoustic_fsafe, cylinder_case
     return cylinder_case(nr, nt, mach=0.8, re=1000.0, fsafe=acoustic_fsafe(0.8))
 
@@ -562,7 +562,7 @@ def main():
     if args.only == "ale":
         print(json.dumps({"ale": ale_compare(args.ale_steps)}))
         return
-    cs = config2_case(args.nr, args.nt)
+    cs = cylinder_bench_case(args.nr, args.nt)
     prec = PRECISION[args.precision]
     rn = not args.no_renumber
     dev, dev_sc, dev_finite = develop(cs, args.develop, prec=0, renumber=rn)

@@ -1,4 +1,4 @@
-"""Parity at the large ALE configuration's size: the ellipse O-grid
+"""Parity on the ellipse O-grid
 4096 x 4096 (16,777,216 nodes, 33,546,240 triangles, Laplace nnz
 117,424,128), M 0.7, Re 10000.
 
@@ -31,10 +31,10 @@ TOL = 1e-12
 
 
 @pytest.fixture(scope="module")
-def big5():
+def big4096():
     orc.set_threads(min(16, os.cpu_count() or 1))
     cs = ellipse_case(4096, 4096, mach=0.7, re=10000.0, moving=0)
-    print(f"config 5 mesh: {cs.npoin} nodes, {cs.nelem} triangles", flush=True)
+    print(f"ellipse 4096 x 4096 mesh: {cs.npoin} nodes, {cs.nelem} triangles", flush=True)
     o = OracleSolver(cs)
     o.step()
     print("oracle: start-up and one step done", flush=True)
@@ -42,9 +42,9 @@ def big5():
     orc.set_threads(1)
 
 
-def test_config5_tables(big5):
+def test_ellipse4096_tables(big4096):
     from cfd_amd.api import MeshContext
-    cs, o = big5
+    cs, o = big4096
     ctx = MeshContext(cs.mesh.inpoel, cs.mesh.npoin)
     t = o.tab
     r, i = ctx.lap_structure()
@@ -59,9 +59,9 @@ def test_config5_tables(big5):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_config5_teacher_forced_step(big5, precision):
+def test_ellipse4096_teacher_forced_step(big4096, precision):
     from cfd_amd.solver import Session
-    cs, o0 = big5
+    cs, o0 = big4096
     st = o0.state()
     o = OracleSolver(cs)
     o.set_state(st)
@@ -86,14 +86,14 @@ def test_config5_teacher_forced_step(big5, precision):
     assert relnorm(ref["U"], st["U"]) > 1e-10
 
 
-def test_config5_relative_pcg(big5):
+def test_ellipse4096_relative_pcg(big4096):
     """The two mesh-Laplace systems of a pitch step at full size, solved under
     the relative rule (||r||_z <= tol ||r0||_z of the guess that is zero off
     the fixed nodes): converged, the rule met by the residual recomputed with
     scipy, the Dirichlet values kept."""
     import scipy.sparse as sp
     from cfd_amd.api import biCG_ex
-    cs, o = big5
+    cs, o = big4096
     s, t = o.s, o.tab
     n = cs.npoin
     m = cs.mesh

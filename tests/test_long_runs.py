@@ -1,5 +1,5 @@
-"""Config 1 at its stated length: the cylinder O-grid 64 x 128, M 0.3,
-Re 100, FSAFE 0.5, 2,000 explicit steps (about ten flow times), on the GPU
+"""The cylinder O-grid 64 x 128 at M 0.3, Re 100, FSAFE 0.5 over
+2,000 explicit steps (about ten flow times), on the GPU
 (fused FP64, reference-order FP64 and FP32-element paths) and on the oracle.
 
 A free run is no per-step parity test: ESTAB's shock-capturing direction is
@@ -39,7 +39,7 @@ def _oracle_run(cs, ulp_node=None):
     return np.array(cd), o.s["U"].copy()
 
 
-def test_config1_full_run():
+def test_cylinder64_full_run():
     from cfd_amd.solver import Session
     cs = cylinder_case(64, 128, mach=0.3, re=100.0)
     cd_ref, U_ref = _oracle_run(cs)
