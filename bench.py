@@ -57,7 +57,9 @@ def element_bytes(nelem: int, npoin: int, precision: int = 0) -> int:
 
     Fused paths (k_patch_stage + k_halo_stage: residual, node sums and the RK
     stage): per element the connectivity 12 B + dNx,dNy + fac,nu,tau1..3 at
-    the element width (FP64: 48 + 40 = 88 B; FP32: 24 + 20 = 44 B); per node
+    the element width (FP64: 48 + 40 = 88 B; FP32: 24 + 20 = 44 B; the six
+    gradients the residual uses, although the kernel reads four and forms
+    the first node's pair as minus the sum of the other two); per node
     U 32 + T 8 + M 8 + GAMM 8 = 56 B read, and written on average over the 4
     stages (3 x T 8 + final U1 32, RHS 32, T, VX, VY, RHO, E, P, RMACH 56) / 4
     = 36 B.

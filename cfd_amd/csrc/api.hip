@@ -665,8 +665,8 @@ struct cfd_session {
     DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, POSAUY, BZERO, partials;
     // fused paths: shape gradients and ESTAB outputs in patch order, binary32
     // (precision 1) or FP64 (precision 0)
-    DBuf<float> DNX32, DNY32, NU32, TA32, TB32, TC32, FAC32;
-    DBuf<double> DNX64, DNY64, NU64, TA64, TB64, TC64, FAC64;
+    DBuf<float> DN32, NU32, TA32, TB32, TC32, FAC32;
+    DBuf<double> DNX64, DNY64, DN64, NU64, TA64, TB64, TC64, FAC64;
     PatchSet patches;
     DBuf<int> ILAUX, ISET;
     DBuf<double> FPART;  // the forces' block partials
@@ -788,11 +788,11 @@ struct cfd_session {
     // the shape gradients read by the fused kernels, in patch order
     void refresh_patch_geometry() {
         if (d.precision == 1) {
-            launch_to_patch(DNX.p, DNX32.p, patches.perm.p, ne, s);
-            launch_to_patch(DNY.p, DNY32.p, patches.perm.p, ne, s);
+            launch_to_patch_grad(DNX.p, DNY.p, DN32.p, patches.perm.p, ne, s);
         } else if (d.precision == 0) {
-            launch_to_patch(DNX.p, DNX64.p, patches.perm.p, ne, s);
+            launch_to_patch(DNX.p, DNX64.p, patches.perm.p, ne, s);  // ESTAB's exact copies
             launch_to_patch(DNY.p, DNY64.p, patches.perm.p, ne, s);
+            launch_to_patch_grad(DNX.p, DNY.p, DN64.p, patches.perm.p, ne, s);
         }
     }
 
@@ -890,9 +890,9 @@ struct cfd_session {
         te(CFD_K_ESTAB);
         ElemIn in{U(), nullptr, T.p, DNX.p, DNY.p, AREA.p, SHOC.p, DTL.p, T1.p, T2.p, T3.p,
                   fu ? WX.p : nullptr, fu ? WY.p : nullptr};
-        PatchIn<float> in32{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DNX32.p, DNY32.p,
+        PatchIn<float> in32{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DN32.p,
                             FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
-        PatchIn<double> in64{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DNX64.p, DNY64.p,
+        PatchIn<double> in64{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DN64.p,
                              FAC64.p, NU64.p, TA64.p, TB64.p, TC64.p};
         StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p, WX.p,
                       WY.p};
@@ -1171,10 +1171,11 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->DNY.alloc(3 * (size_t)ne);
         S->LAP.alloc(m->t.nnz);
         if (d->precision == 1) {
-            for (auto* b : {&S->DNX32, &S->DNY32}) b->alloc(3 * (size_t)ne);
+            S->DN32.alloc(4 * (size_t)ne);
             for (auto* b : {&S->NU32, &S->TA32, &S->TB32, &S->TC32, &S->FAC32}) b->alloc(ne);
         } else if (d->precision == 0) {
             for (auto* b : {&S->DNX64, &S->DNY64}) b->alloc(3 * (size_t)ne);
+            S->DN64.alloc(4 * (size_t)ne);
             for (auto* b : {&S->NU64, &S->TA64, &S->TB64, &S->TC64, &S->FAC64}) b->alloc(ne);
         }
         if (S->fused()) {

@@ -42,8 +42,8 @@ struct cfd_batch {
     int ucur = 0;
     DBuf<double> RHS, T, VX, VY, WX, WY, X, Y, RHO, E, P, RMACH, M, GAMM;
     DBuf<double> DTL, DT, AREA, DNX, DNY, HH, HHX, HHY, SHOC, T1, T2, T3;
-    DBuf<float> DNX32, DNY32, NU32, TA32, TB32, TC32, FAC32;
-    DBuf<double> DNX64, DNY64, NU64, TA64, TB64, TC64, FAC64;
+    DBuf<float> DN32, NU32, TA32, TB32, TC32, FAC32;
+    DBuf<double> DNX64, DNY64, DN64, NU64, TA64, TB64, TC64, FAC64;
     DBuf<int> ecase, chunk, chunk_case, pb;
     DBuf<Consts> cases;  // per case
     DBuf<double> ctl, partials, hmin;
@@ -96,11 +96,11 @@ struct cfd_batch {
 
     void refresh_patch_geometry() {
         if (precision == 1) {
-            launch_to_patch(DNX.p, DNX32.p, patches.perm.p, ne, s);
-            launch_to_patch(DNY.p, DNY32.p, patches.perm.p, ne, s);
+            launch_to_patch_grad(DNX.p, DNY.p, DN32.p, patches.perm.p, ne, s);
         } else {
-            launch_to_patch(DNX.p, DNX64.p, patches.perm.p, ne, s);
+            launch_to_patch(DNX.p, DNX64.p, patches.perm.p, ne, s);  // ESTAB's exact copies
             launch_to_patch(DNY.p, DNY64.p, patches.perm.p, ne, s);
+            launch_to_patch_grad(DNX.p, DNY.p, DN64.p, patches.perm.p, ne, s);
         }
     }
 
@@ -139,9 +139,9 @@ struct cfd_batch {
         else
             launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
                                DNY.p, DNX64.p, DNY64.p, dtmin, elem64(), s);
-        PatchIn<float> in32{U(), T.p, nullptr, nullptr, DNX32.p, DNY32.p,
+        PatchIn<float> in32{U(), T.p, nullptr, nullptr, DN32.p,
                             FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
-        PatchIn<double> in64{U(), T.p, nullptr, nullptr, DNX64.p, DNY64.p,
+        PatchIn<double> in64{U(), T.p, nullptr, nullptr, DN64.p,
                              FAC64.p, NU64.p, TA64.p, TB64.p, TC64.p};
         StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p,
                       WX.p, WY.p};
@@ -367,10 +367,11 @@ int cfd_batch_create(cfd_batch** out, int ncases, const cfd_batch_case* cases, i
         B->DNX.alloc(3 * (size_t)ne);
         B->DNY.alloc(3 * (size_t)ne);
         if (precision == 1) {
-            for (auto* b : {&B->DNX32, &B->DNY32}) b->alloc(3 * (size_t)ne);
+            B->DN32.alloc(4 * (size_t)ne);
             for (auto* b : {&B->NU32, &B->TA32, &B->TB32, &B->TC32, &B->FAC32}) b->alloc(ne);
         } else {
             for (auto* b : {&B->DNX64, &B->DNY64}) b->alloc(3 * (size_t)ne);
+            B->DN64.alloc(4 * (size_t)ne);
             for (auto* b : {&B->NU64, &B->TA64, &B->TB64, &B->TC64, &B->FAC64}) b->alloc(ne);
         }
         for (auto* b : {&B->Ub[0], &B->Ub[1], &B->RHS, &B->P, &B->RMACH, &B->DTL, &B->DT})

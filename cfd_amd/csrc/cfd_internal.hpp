@@ -353,7 +353,11 @@ void build_patches_batch(PatchSet& ps, int ncases, const int* const* inpoel_host
 template <class R>
 struct PatchIn {
     const double *U, *T, *WX, *WY;
-    const R *dNx, *dNy, *fac, *nu, *ta, *tb, *tc;
+    // per element 4 values (Nx2, Nx3, Ny2, Ny3: the gradients of the second
+    // and third shape functions; the first is minus their sum), 16- or
+    // 32-byte aligned
+    const R* dN;
+    const R *fac, *nu, *ta, *tb, *tc;
 };
 template <class R>
 void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, const BcTable& bc,
@@ -370,6 +374,11 @@ bool patch_lds_fits(int max_nodes, int pe, size_t slot_bytes);
 // dNx/dNy in patch order, rounded to R
 template <class R>
 void launch_to_patch(const double* a, R* b, const int* perm, int nelem, hipStream_t s);
+// The fused kernel's gradients in patch order, rounded to R: 4 per element
+// (Nx2, Nx3, Ny2, Ny3 of the element at each patch position)
+template <class R>
+void launch_to_patch_grad(const double* dNx, const double* dNy, R* dN, const int* perm, int nelem,
+                          hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // MLP on the matrix cores (mlp.hip): 16 -> 64 -> 64 -> 16, tanh, tanh,

@@ -855,8 +855,10 @@ k_patch_stage(PatchDev pd, ElemConstsP<R> kc, PatchIn<R> in, StageDev a) {
             const int ln[3] = {(int)(w & 1023u), (int)((w >> 10) & 1023u), (int)((w >> 20) & 1023u)};
             const int ls[3] = {(int)((w >> 30) & 2047u), (int)((w >> 41) & 2047u),
                                (int)((w >> 52) & 2047u)};
-            const R Nx[3] = {in.dNx[3 * e], in.dNx[3 * e + 1], in.dNx[3 * e + 2]};
-            const R Ny[3] = {in.dNy[3 * e], in.dNy[3 * e + 1], in.dNy[3 * e + 2]};
+            // P1 gradients sum to zero: the first from the other two
+            const V4 gN = *reinterpret_cast<const V4*>(in.dN + 4 * e);
+            const R Nx[3] = {-(gN.x + gN.y), gN.x, gN.y};
+            const R Ny[3] = {-(gN.z + gN.w), gN.z, gN.w};
             const R tau[3] = {in.ta[e], in.tb[e], in.tc[e]};
             const double T3[3] = {sT[ln[0]], sT[ln[1]], sT[ln[2]]};
             double wx3[3] = {0.0, 0.0, 0.0}, wy3[3] = {0.0, 0.0, 0.0};
@@ -1014,6 +1016,16 @@ __global__ void k_to_patch(const double* __restrict__ a, R* __restrict__ b,
     const long long e = perm[q];
 #pragma unroll
     for (int k = 0; k < 3; ++k) b[3 * (long long)q + k] = (R)a[3 * e + k];
+}
+
+template <class R>
+__global__ void k_to_patch_grad(const double* __restrict__ nx, const double* __restrict__ ny,
+                                R* __restrict__ dN, const int* __restrict__ perm, int nelem) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nelem) return;
+    const long long e = perm[q];
+    *reinterpret_cast<typename Vec4<R>::type*>(dN + 4 * (long long)q) =
+        make4((R)nx[3 * e + 1], (R)nx[3 * e + 2], (R)ny[3 * e + 1], (R)ny[3 * e + 2]);
 }
 
 }  // namespace
@@ -1315,5 +1327,17 @@ template void launch_patch_stage<double>(const Tables&, const PatchSet&, const C
                                          const StageNodes&, bool, hipStream_t);
 template void launch_to_patch<float>(const double*, float*, const int*, int, hipStream_t);
 template void launch_to_patch<double>(const double*, double*, const int*, int, hipStream_t);
+
+template <class R>
+void launch_to_patch_grad(const double* dNx, const double* dNy, R* dN, const int* perm, int nelem,
+                          hipStream_t s) {
+    if (nelem <= 0) return;
+    k_to_patch_grad<R><<<grid_for(nelem), kBlock, 0, s>>>(dNx, dNy, dN, perm, nelem);
+    CFD_HIP(hipGetLastError());
+}
+template void launch_to_patch_grad<float>(const double*, const double*, float*, const int*, int,
+                                          hipStream_t);
+template void launch_to_patch_grad<double>(const double*, const double*, double*, const int*, int,
+                                           hipStream_t);
 
 }  // namespace cfd

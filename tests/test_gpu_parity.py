@@ -663,35 +663,44 @@ def test_learned_warm_start_cuts_mesh_solve_iterations():
     assert diff.max() < 0.1                             # the same flow
 
 
-@pytest.mark.parametrize("tol", [1e-7, 1e-8])
-def test_trained_network_gains_over_the_extrapolation_alone(tol):
+def test_trained_network_gains_over_the_extrapolation_alone():
     """What the network adds: the same moving-ellipse run (FP32 elements)
     with the learned start (extrapolation + the network's Galerkin-scaled
     correction) and with the extrapolation alone (warm_start 2, no network),
-    both under the same relative PCG rule (measured ratios 0.73 and 0.76).  From the fourth step on (the first with the network)
-    the learned start needs clearly fewer mesh-solve iterations, and both
-    meshes agree with each other to the rule's accuracy."""
+    both under the same relative PCG rule, at 1e-7 and 1e-8, over 120 steps.
+    From the fourth step on (the first with the network) the learned start
+    needs clearly fewer mesh-solve iterations (measured, on two builds whose
+    flows differ at round-off: 0.71-0.80 of the extrapolation's per rule,
+    0.72-0.74 pooled), and both meshes agree with each other to the rule's
+    accuracy.  The ratio is taken over a long run because over a few dozen
+    steps it scatters (0.71-0.97 measured): DTMIN's 1 % hysteresis jumps,
+    which follow the flow's round-off, set the extrapolation's error."""
     from cfd_amd.solver import Session
     cs = ellipse_case(48, 96)
-    nsteps, n0 = 30, 3
-    out = {}
-    for ws in (2, 1):
-        S = Session(cs, precision=1, pcg_rel=1, pcg_tol=tol, warm_start=ws)
-        S.step(n0)
-        it0 = S.scalars()["PCG_ITSUM"]
-        S.step(nsteps - n0)
-        sc = S.scalars()
-        out[ws] = dict(its=sc["PCG_ITSUM"] - it0, sc=sc, X=S.get("X"), Y=S.get("Y"))
-        S.close()
-    e, q = out[2], out[1]
-    print(f"tol {tol:g}: iterations after step {n0}: extrapolation {e['its']:.0f}, "
-          f"learned {q['its']:.0f} (loss {q['sc']['WS_LOSS']:.3e})")
-    assert e["sc"]["WS_SOLVES"] == q["sc"]["WS_SOLVES"] == 2 * (nsteps - 2)
-    assert q["its"] <= 0.85 * e["its"]
-    assert np.isfinite(q["sc"]["WS_LOSS"]) and q["sc"]["WS_LOSS"] < 1.0
-    disp = np.abs(e["X"] - cs.mesh.x).max() + np.abs(e["Y"] - cs.mesh.y).max()
-    dx = max(np.abs(q["X"] - e["X"]).max(), np.abs(q["Y"] - e["Y"]).max())
-    assert disp > 0 and dx <= 1e-2 * disp
+    nsteps, n0 = 120, 3
+    tot = {1: 0.0, 2: 0.0}
+    for tol in (1e-7, 1e-8):
+        out = {}
+        for ws in (2, 1):
+            S = Session(cs, precision=1, pcg_rel=1, pcg_tol=tol, warm_start=ws)
+            S.step(n0)
+            it0 = S.scalars()["PCG_ITSUM"]
+            S.step(nsteps - n0)
+            sc = S.scalars()
+            out[ws] = dict(its=sc["PCG_ITSUM"] - it0, sc=sc, X=S.get("X"), Y=S.get("Y"))
+            tot[ws] += out[ws]["its"]
+            S.close()
+        e, q = out[2], out[1]
+        print(f"tol {tol:g}: iterations after step {n0}: extrapolation {e['its']:.0f}, "
+              f"learned {q['its']:.0f} (loss {q['sc']['WS_LOSS']:.3e})")
+        assert e["sc"]["WS_SOLVES"] == q["sc"]["WS_SOLVES"] == 2 * (nsteps - 2)
+        assert q["its"] <= 0.9 * e["its"]
+        assert np.isfinite(q["sc"]["WS_LOSS"]) and q["sc"]["WS_LOSS"] < 1.0
+        disp = np.abs(e["X"] - cs.mesh.x).max() + np.abs(e["Y"] - cs.mesh.y).max()
+        dx = max(np.abs(q["X"] - e["X"]).max(), np.abs(q["Y"] - e["Y"]).max())
+        assert disp > 0 and dx <= 1e-2 * disp
+    print(f"pooled ratio {tot[1] / tot[2]:.3f}")
+    assert tot[1] <= 0.85 * tot[2]
 
 
 def test_learned_warm_start_gate_under_the_absolute_rule():
