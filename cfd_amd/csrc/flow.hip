@@ -806,7 +806,7 @@ struct PatchDev {
 // At most 168 VGPRs, so that 3 waves per SIMD stay resident (the LDS of the
 // default patch shapes allows 3 blocks of 256 threads per CU): the FP64
 // fixed-mesh stages fit without spilling (174 VGPRs would leave 2 waves),
-// the FP64 moving-mesh stages spill 40 B per thread and still run faster
+// the FP64 moving-mesh stages spill 24 B per thread and still run faster
 // than at 188 VGPRs and 2 waves
 template <class R, bool kFuente, bool kFinal>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3)))
