@@ -371,9 +371,8 @@ void patch_make_persistent(PatchSet& ps, bool fuente);
 // slot ranges) fits its 64 KB budget for patches of pe elements and up to
 // max_nodes nodes
 bool patch_lds_fits(int max_nodes, int pe, size_t slot_bytes);
-// dNx/dNy in patch order, rounded to R
-template <class R>
-void launch_to_patch(const double* a, R* b, const int* perm, int nelem, hipStream_t s);
+// dNx/dNy in patch order, exact (ESTAB's inputs on the fused FP64 path)
+void launch_to_patch(const double* a, double* b, const int* perm, int nelem, hipStream_t s);
 // The fused kernel's gradients in patch order, rounded to R: 4 per element
 // (Nx2, Nx3, Ny2, Ny3 of the element at each patch position)
 template <class R>

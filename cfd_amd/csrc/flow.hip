@@ -1007,15 +1007,14 @@ k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
     }
 }
 
-// b[3q+k] = (R)a[3 perm[q] + k]
-template <class R>
-__global__ void k_to_patch(const double* __restrict__ a, R* __restrict__ b,
+// b[3q+k] = a[3 perm[q] + k]
+__global__ void k_to_patch(const double* __restrict__ a, double* __restrict__ b,
                            const int* __restrict__ perm, int nelem) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nelem) return;
     const long long e = perm[q];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) b[3 * (long long)q + k] = (R)a[3 * e + k];
+    for (int k = 0; k < 3; ++k) b[3 * (long long)q + k] = a[3 * e + k];
 }
 
 template <class R>
@@ -1312,10 +1311,9 @@ void patch_make_persistent(PatchSet& ps, bool fuente) {
 template void patch_make_persistent<float>(PatchSet&, bool);
 template void patch_make_persistent<double>(PatchSet&, bool);
 
-template <class R>
-void launch_to_patch(const double* a, R* b, const int* perm, int nelem, hipStream_t s) {
+void launch_to_patch(const double* a, double* b, const int* perm, int nelem, hipStream_t s) {
     if (nelem <= 0) return;
-    k_to_patch<R><<<grid_for(nelem), kBlock, 0, s>>>(a, b, perm, nelem);
+    k_to_patch<<<grid_for(nelem), kBlock, 0, s>>>(a, b, perm, nelem);
     CFD_HIP(hipGetLastError());
 }
 
@@ -1325,8 +1323,6 @@ template void launch_patch_stage<float>(const Tables&, const PatchSet&, const Co
 template void launch_patch_stage<double>(const Tables&, const PatchSet&, const Consts&,
                                          const BcTable&, double, const PatchIn<double>&,
                                          const StageNodes&, bool, hipStream_t);
-template void launch_to_patch<float>(const double*, float*, const int*, int, hipStream_t);
-template void launch_to_patch<double>(const double*, double*, const int*, int, hipStream_t);
 
 template <class R>
 void launch_to_patch_grad(const double* dNx, const double* dNy, R* dN, const int* perm, int nelem,
