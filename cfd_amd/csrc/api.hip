@@ -859,9 +859,13 @@ struct cfd_session {
         tb(CFD_K_STEP);
         // deltat, DTMIN with its hysteresis, DTL, the clock (ns2DComp.ALE.f90:142-166)
         tb(CFD_K_DELTAT);
-        // fused paths: in patch order with the patch's nodes in LDS (the same DT)
+        // fused paths: in patch order with the patch's nodes in LDS (the same DT);
+        // W not read while it is zero (no moving nodes, W never set)
+        const bool fu = fuente_needed();
+        const double* wx = fu ? WX.p : nullptr;
+        const double* wy = fu ? WY.p : nullptr;
         const int nb = fused() && patches.npatch <= (int)partials.n
-                           ? launch_deltat_patch(patches, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p,
+                           ? launch_deltat_patch(patches, c, AREA.p, T.p, VX.p, VY.p, wx, wy,
                                                  DT.p, partials.p, s)
                            : launch_deltat_partials(t, c, AREA.p, T.p, VX.p, VY.p, WX.p, WY.p,
                                                     DT.p, partials.p, s);
@@ -876,13 +880,12 @@ struct cfd_session {
         const double* dtmin = scal(CFD_S_DTMIN);
 
         // RK (subrutinas.f90:645-849)
-        const bool fu = fuente_needed();
         tb(CFD_K_ESTAB);
         if (d.precision == 1)
-            launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
+            launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, wx, wy, DNX.p,
                                DNY.p, nullptr, nullptr, dtmin, elem32(), s);
         else if (d.precision == 0)
-            launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
+            launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, wx, wy, DNX.p,
                                DNY.p, DNX64.p, DNY64.p, dtmin, elem64(), s);
         else
             launch_estab(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,

@@ -108,7 +108,8 @@ k_deltat_patch(const int* __restrict__ node_base, const int* __restrict__ elem_b
     const int nb = node_base[p], nl = node_base[p + 1] - nb;
     for (int l = threadIdx.x; l < nl; l += blockDim.x) {
         const int g = pnode[nb + l] - 1;
-        sN[l] = DtNode{T[g], fabs(VX[g] - WX[g]), fabs(VY[g] - WY[g])};
+        // null WX/WY: a mesh at rest (W = 0, and v - 0 = v exactly)
+        sN[l] = DtNode{T[g], fabs(VX[g] - (WX ? WX[g] : 0.0)), fabs(VY[g] - (WY ? WY[g] : 0.0))};
     }
     __syncthreads();
     const long long e0 = elem_base[p];
@@ -945,7 +946,7 @@ struct EstabNode {
     double rho, T, vx, vy, wx, wy, gm;
 };
 struct EstabIn {
-    const double *U, *T, *GAMM, *VX, *VY, *WX, *WY;
+    const double *U, *T, *GAMM, *VX, *VY, *WX, *WY;  // WX, WY null: W = 0
     const double *dNx, *dNy;      // element order
     const double *dNxp, *dNyp;    // patch order (null: read dNx/dNy through perm)
     const int* perm;
@@ -971,7 +972,7 @@ k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
         for (int l = threadIdx.x; l < nl; l += blockDim.x) {
             const int g = pd.pnode[nb + l] - 1;
             sN[l] = EstabNode{ei.U[4 * (long long)g], ei.T[g], ei.VX[g], ei.VY[g],
-                              ei.WX[g], ei.WY[g], ei.GAMM[g]};
+                              ei.WX ? ei.WX[g] : 0.0, ei.WY ? ei.WY[g] : 0.0, ei.GAMM[g]};
         }
         __syncthreads();
         for (int q = threadIdx.x; q < nep; q += blockDim.x) {
