@@ -468,6 +468,36 @@ def test_session_teacher_forced(which, precision, request):
         assert_forces_match(cs, st, ref, got)
 
 
+def test_mesh_at_rest_skips_the_w_reads(cyl):
+    """On a fixed mesh whose W the caller never set (W = 0), the patch-order
+    deltat and ESTAB do not read W (v - 0 = v exactly): a step from the
+    oracle's state loaded without W lands within 1e-12 of the oracle, and its
+    DT, SHOC and T_SUGN1..3 equal bit for bit those of the same step on a
+    session that loaded W = 0 and therefore reads it."""
+    from cfd_amd.solver import FIELD_OF, Session
+    cs, o0 = cyl
+    st = o0.state()
+    assert np.abs(st["W_X"]).max() == 0 and np.abs(st["W_Y"]).max() == 0
+    no_w = [k for k in FIELD_OF if k in st and k not in ("W_X", "W_Y", "XPOS", "YPOS")]
+    o = OracleSolver(cs)
+    o.set_state(st)
+    o.step()
+    ref = o.state()
+    out = {}
+    for tag, fields in (("skip", no_w), ("read", no_w + ["W_X", "W_Y"])):
+        S = Session(cs, precision=0)
+        S.set_state(st, fields=fields)
+        S.step(1)
+        out[tag] = S.state() | {k: S.get(k) for k in ("DT", "SHOC", "T_SUGN1", "T_SUGN2", "T_SUGN3")}
+        S.close()
+    got = out["skip"]
+    assert got["DTMIN"] == ref["DTMIN"]
+    assert np.all(relnorm_cols(got["U"], ref["U"]) < TOL), relnorm_cols(got["U"], ref["U"])
+    assert np.all(relnorm_cols(got["RHS"], ref["RHS"]) < TOL), relnorm_cols(got["RHS"], ref["RHS"])
+    for f in ("DT", "SHOC", "T_SUGN1", "T_SUGN2", "T_SUGN3"):
+        assert bitdiff(out["skip"][f], out["read"][f]) == 0, f
+
+
 def assert_forces_match(cs, st, ref, got, tol=TOL):
     """FORCES (meshMove.f90:153-194) on the GPU against the oracle after a
     teacher-forced step: FX, FY and RM within tol of the sum of the
