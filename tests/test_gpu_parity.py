@@ -468,10 +468,12 @@ def test_session_teacher_forced(which, precision, request):
         assert_forces_match(cs, st, ref, got)
 
 
-def test_mesh_at_rest_skips_the_w_reads(cyl):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_mesh_at_rest_skips_the_w_reads(cyl, precision):
     """On a fixed mesh whose W the caller never set (W = 0), the patch-order
     deltat and ESTAB do not read W (v - 0 = v exactly): a step from the
-    oracle's state loaded without W lands within 1e-12 of the oracle, and its
+    oracle's state loaded without W lands within the path's bar of the
+    oracle (1e-12 for the fused FP64 path, 2e-4 for FP32 elements), and its
     DT, SHOC and T_SUGN1..3 equal bit for bit those of the same step on a
     session that loaded W = 0 and therefore reads it."""
     from cfd_amd.solver import FIELD_OF, Session
@@ -485,15 +487,16 @@ def test_mesh_at_rest_skips_the_w_reads(cyl):
     ref = o.state()
     out = {}
     for tag, fields in (("skip", no_w), ("read", no_w + ["W_X", "W_Y"])):
-        S = Session(cs, precision=0)
+        S = Session(cs, precision=precision)
         S.set_state(st, fields=fields)
         S.step(1)
-        out[tag] = S.state() | {k: S.get(k) for k in ("DT", "SHOC", "T_SUGN1", "T_SUGN2", "T_SUGN3")}
+        out[tag] = S.state()
         S.close()
     got = out["skip"]
+    tol = TOL if precision == 0 else 2e-4
     assert got["DTMIN"] == ref["DTMIN"]
-    assert np.all(relnorm_cols(got["U"], ref["U"]) < TOL), relnorm_cols(got["U"], ref["U"])
-    assert np.all(relnorm_cols(got["RHS"], ref["RHS"]) < TOL), relnorm_cols(got["RHS"], ref["RHS"])
+    assert np.all(relnorm_cols(got["U"], ref["U"]) < tol), relnorm_cols(got["U"], ref["U"])
+    assert np.all(relnorm_cols(got["RHS"], ref["RHS"]) < tol), relnorm_cols(got["RHS"], ref["RHS"])
     for f in ("DT", "SHOC", "T_SUGN1", "T_SUGN2", "T_SUGN3"):
         assert bitdiff(out["skip"][f], out["read"][f]) == 0, f
 
