@@ -666,7 +666,7 @@ struct cfd_session {
     // fused paths: shape gradients and ESTAB outputs in patch order, binary32
     // (precision 1) or FP64 (precision 0)
     DBuf<float> DN32, NU32, TA32, TB32, TC32, FAC32;
-    DBuf<double> DNX64, DNY64, DN64, NU64, TA64, TB64, TC64, FAC64;
+    DBuf<double> DN6, DN64, NU64, TA64, TB64, TC64, FAC64;
     PatchSet patches;
     DBuf<int> ILAUX, ISET;
     DBuf<double> FPART;  // the forces' block partials
@@ -790,8 +790,7 @@ struct cfd_session {
         if (d.precision == 1) {
             launch_to_patch_grad(DNX.p, DNY.p, DN32.p, patches.perm.p, ne, s);
         } else if (d.precision == 0) {
-            launch_to_patch(DNX.p, DNX64.p, patches.perm.p, ne, s);  // ESTAB's exact copies
-            launch_to_patch(DNY.p, DNY64.p, patches.perm.p, ne, s);
+            launch_to_patch6(DNX.p, DNY.p, DN6.p, patches.perm.p, ne, s);  // ESTAB's exact copy
             launch_to_patch_grad(DNX.p, DNY.p, DN64.p, patches.perm.p, ne, s);
         }
     }
@@ -883,10 +882,10 @@ struct cfd_session {
         tb(CFD_K_ESTAB);
         if (d.precision == 1)
             launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, wx, wy, DNX.p,
-                               DNY.p, nullptr, nullptr, dtmin, elem32(), s);
+                               DNY.p, nullptr, dtmin, elem32(), s);
         else if (d.precision == 0)
             launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, wx, wy, DNX.p,
-                               DNY.p, DNX64.p, DNY64.p, dtmin, elem64(), s);
+                               DNY.p, DN6.p, dtmin, elem64(), s);
         else
             launch_estab(t, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p, DNY.p, dtmin,
                          SHOC.p, T1.p, T2.p, T3.p, s);
@@ -1177,7 +1176,7 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
             S->DN32.alloc(4 * (size_t)ne);
             for (auto* b : {&S->NU32, &S->TA32, &S->TB32, &S->TC32, &S->FAC32}) b->alloc(ne);
         } else if (d->precision == 0) {
-            for (auto* b : {&S->DNX64, &S->DNY64}) b->alloc(3 * (size_t)ne);
+            S->DN6.alloc(6 * (size_t)ne);
             S->DN64.alloc(4 * (size_t)ne);
             for (auto* b : {&S->NU64, &S->TA64, &S->TB64, &S->TC64, &S->FAC64}) b->alloc(ne);
         }

@@ -43,7 +43,7 @@ struct cfd_batch {
     DBuf<double> RHS, T, VX, VY, WX, WY, X, Y, RHO, E, P, RMACH, M, GAMM;
     DBuf<double> DTL, DT, AREA, DNX, DNY, HH, HHX, HHY, SHOC, T1, T2, T3;
     DBuf<float> DN32, NU32, TA32, TB32, TC32, FAC32;
-    DBuf<double> DNX64, DNY64, DN64, NU64, TA64, TB64, TC64, FAC64;
+    DBuf<double> DN6, DN64, NU64, TA64, TB64, TC64, FAC64;
     DBuf<int> ecase, chunk, chunk_case, pb;
     DBuf<Consts> cases;  // per case
     DBuf<double> ctl, partials, hmin;
@@ -98,8 +98,7 @@ struct cfd_batch {
         if (precision == 1) {
             launch_to_patch_grad(DNX.p, DNY.p, DN32.p, patches.perm.p, ne, s);
         } else {
-            launch_to_patch(DNX.p, DNX64.p, patches.perm.p, ne, s);  // ESTAB's exact copies
-            launch_to_patch(DNY.p, DNY64.p, patches.perm.p, ne, s);
+            launch_to_patch6(DNX.p, DNY.p, DN6.p, patches.perm.p, ne, s);  // ESTAB's exact copy
             launch_to_patch_grad(DNX.p, DNY.p, DN64.p, patches.perm.p, ne, s);
         }
     }
@@ -135,10 +134,10 @@ struct cfd_batch {
         launch_dt_finish(DT.p, DTL.p, ctl.p, ecase.p, itlocal != 0, ne, s);
         if (precision == 1)
             launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
-                               DNY.p, nullptr, nullptr, dtmin, elem32(), s);
+                               DNY.p, nullptr, dtmin, elem32(), s);
         else
             launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
-                               DNY.p, DNX64.p, DNY64.p, dtmin, elem64(), s);
+                               DNY.p, DN6.p, dtmin, elem64(), s);
         PatchIn<float> in32{U(), T.p, nullptr, nullptr, DN32.p,
                             FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
         PatchIn<double> in64{U(), T.p, nullptr, nullptr, DN64.p,
@@ -370,7 +369,7 @@ int cfd_batch_create(cfd_batch** out, int ncases, const cfd_batch_case* cases, i
             B->DN32.alloc(4 * (size_t)ne);
             for (auto* b : {&B->NU32, &B->TA32, &B->TB32, &B->TC32, &B->FAC32}) b->alloc(ne);
         } else {
-            for (auto* b : {&B->DNX64, &B->DNY64}) b->alloc(3 * (size_t)ne);
+            B->DN6.alloc(6 * (size_t)ne);
             B->DN64.alloc(4 * (size_t)ne);
             for (auto* b : {&B->NU64, &B->TA64, &B->TB64, &B->TC64, &B->FAC64}) b->alloc(ne);
         }

@@ -247,15 +247,16 @@ void launch_estab(const Tables& t, const Consts& c, const double* U, const doubl
                   double* shoc, double* t1, double* t2, double* t3, hipStream_t s);
 // The same arithmetic in patch order, writing the patch-order inputs o of
 // the fused kernel: one block per patch of ps with the patch's node values
-// in LDS.  dNx_patch/dNy_patch: the patch-order FP64 gradients when they
-// exist (the fused FP64 path), else null (read dNx/dNy through ps.perm).
+// in LDS.  dN6_patch: the patch-order FP64 gradients, 6 per element (Nx1..3
+// then Ny1..3), when they exist (the fused FP64 path), else null (read
+// dNx/dNy through ps.perm).
 struct PatchSet;
 template <class R>
 void launch_estab_patch(const Tables& t, const PatchSet& ps, const Consts& c, const double* U,
                         const double* T, const double* GAMM, const double* VX, const double* VY,
                         const double* WX, const double* WY, const double* dNx, const double* dNy,
-                        const double* dNx_patch, const double* dNy_patch, const double* dtmin,
-                        const ElemP<R>& o, hipStream_t s);
+                        const double* dN6_patch, const double* dtmin, const ElemP<R>& o,
+                        hipStream_t s);
 // SHOC, T_SUGN1..3 (element order, FP64) from the patch-order copies of o
 template <class R>
 void launch_estab_widen(const ElemP<R>& o, const int* inv_perm, int nelem, double* shoc,
@@ -371,8 +372,10 @@ void patch_make_persistent(PatchSet& ps, bool fuente);
 // slot ranges) fits its 64 KB budget for patches of pe elements and up to
 // max_nodes nodes
 bool patch_lds_fits(int max_nodes, int pe, size_t slot_bytes);
-// dNx/dNy in patch order, exact (ESTAB's inputs on the fused FP64 path)
-void launch_to_patch(const double* a, double* b, const int* perm, int nelem, hipStream_t s);
+// dNx/dNy in patch order, exact, 6 per element (Nx1..3 then Ny1..3: ESTAB's
+// inputs on the fused FP64 path)
+void launch_to_patch6(const double* dNx, const double* dNy, double* dN6, const int* perm,
+                      int nelem, hipStream_t s);
 // The fused kernel's gradients in patch order, rounded to R: 4 per element
 // (Nx2, Nx3, Ny2, Ny3 of the element at each patch position)
 template <class R>

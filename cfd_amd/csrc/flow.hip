@@ -948,7 +948,7 @@ struct EstabNode {
 struct EstabIn {
     const double *U, *T, *GAMM, *VX, *VY, *WX, *WY;  // WX, WY null: W = 0
     const double *dNx, *dNy;      // element order
-    const double *dNxp, *dNyp;    // patch order (null: read dNx/dNy through perm)
+    const double* dN6;            // patch order, 6 per element (null: dNx/dNy through perm)
     const int* perm;
     const double* dtmin;          // + case * dt_stride
     double fr, tinf, rinf;
@@ -981,10 +981,24 @@ k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
             const int ln[3] = {(int)(w & 1023u), (int)((w >> 10) & 1023u), (int)((w >> 20) & 1023u)};
             const int eo = ei.perm[e];
             double DNX[3], DNY[3], rho[3], Tn[3], vxn[3], vyn[3], wxn[3], wyn[3], gmn[3];
+            if (ei.dN6) {  // one 48-byte record: three 16-byte loads
+                const double2* r = reinterpret_cast<const double2*>(ei.dN6 + 6 * e);
+                const double2 a = r[0], b = r[1], c = r[2];
+                DNX[0] = a.x;
+                DNX[1] = a.y;
+                DNX[2] = b.x;
+                DNY[0] = b.y;
+                DNY[1] = c.x;
+                DNY[2] = c.y;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    DNX[k] = ei.dNx[3 * (long long)eo + k];
+                    DNY[k] = ei.dNy[3 * (long long)eo + k];
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                DNX[k] = ei.dNxp ? ei.dNxp[3 * e + k] : ei.dNx[3 * (long long)eo + k];
-                DNY[k] = ei.dNyp ? ei.dNyp[3 * e + k] : ei.dNy[3 * (long long)eo + k];
                 const EstabNode& nd = sN[ln[k]];
                 rho[k] = nd.rho;
                 Tn[k] = nd.T;
@@ -1008,14 +1022,17 @@ k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
     }
 }
 
-// b[3q+k] = a[3 perm[q] + k]
-__global__ void k_to_patch(const double* __restrict__ a, double* __restrict__ b,
-                           const int* __restrict__ perm, int nelem) {
+// b[6q+k] = nx[3 perm[q] + k], b[6q+3+k] = ny[3 perm[q] + k]
+__global__ void k_to_patch6(const double* __restrict__ nx, const double* __restrict__ ny,
+                            double* __restrict__ b, const int* __restrict__ perm, int nelem) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nelem) return;
     const long long e = perm[q];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) b[3 * (long long)q + k] = a[3 * e + k];
+    for (int k = 0; k < 3; ++k) {
+        b[6 * (long long)q + k] = nx[3 * e + k];
+        b[6 * (long long)q + 3 + k] = ny[3 * e + k];
+    }
 }
 
 template <class R>
@@ -1098,7 +1115,7 @@ template <class R>
 void launch_estab_patch(const Tables& t, const PatchSet& ps, const Consts& c, const double* U,
                         const double* T, const double* GAMM, const double* VX, const double* VY,
                         const double* WX, const double* WY, const double* dNx, const double* dNy,
-                        const double* dNx_patch, const double* dNy_patch, const double* dtmin,
+                        const double* dN6_patch, const double* dtmin,
                         const ElemP<R>& o, hipStream_t s) {
     CFD_CHECK(o.nu && ps.perm.p, "internal: patch outputs missing");
     CFD_CHECK(ps.max_nodes <= kPatchMaxNodes, "internal: patch beyond the node budget");
@@ -1113,7 +1130,7 @@ void launch_estab_patch(const Tables& t, const PatchSet& ps, const Consts& c, co
     pd.npatch = ps.npatch;
     pd.pcase = ps.cases ? ps.pcase.p : nullptr;
     pd.cases = ps.cases;
-    EstabIn ei{U, T, GAMM, VX, VY, WX, WY, dNx, dNy, dNx_patch, dNy_patch, ps.perm.p, dtmin,
+    EstabIn ei{U, T, GAMM, VX, VY, WX, WY, dNx, dNy, dN6_patch, ps.perm.p, dtmin,
                c.fr, c.t_inf, c.rho_inf};
     const size_t lds = (size_t)ps.max_nodes * sizeof(EstabNode);
     k_estab_patch<R><<<ps.npatch, patch_pass_block(ps.pe), lds, s>>>(pd, ei, o);
@@ -1132,14 +1149,14 @@ template void launch_estab_patch<float>(const Tables&, const PatchSet&, const Co
                                          const double*, const double*, const double*,
                                          const double*, const double*, const double*,
                                          const double*, const double*, const double*,
-                                         const double*, const double*, const double*,
-                                         const ElemP<float>&, hipStream_t);
+                                         const double*, const double*, const ElemP<float>&,
+                                         hipStream_t);
 template void launch_estab_patch<double>(const Tables&, const PatchSet&, const Consts&,
                                           const double*, const double*, const double*,
                                           const double*, const double*, const double*,
                                           const double*, const double*, const double*,
-                                          const double*, const double*, const double*,
-                                          const ElemP<double>&, hipStream_t);
+                                          const double*, const double*, const ElemP<double>&,
+                                          hipStream_t);
 template void launch_estab_widen<float>(const ElemP<float>&, const int*, int, double*, double*,
                                         double*, double*, hipStream_t);
 template void launch_estab_widen<double>(const ElemP<double>&, const int*, int, double*, double*,
@@ -1312,9 +1329,10 @@ void patch_make_persistent(PatchSet& ps, bool fuente) {
 template void patch_make_persistent<float>(PatchSet&, bool);
 template void patch_make_persistent<double>(PatchSet&, bool);
 
-void launch_to_patch(const double* a, double* b, const int* perm, int nelem, hipStream_t s) {
+void launch_to_patch6(const double* dNx, const double* dNy, double* dN6, const int* perm,
+                      int nelem, hipStream_t s) {
     if (nelem <= 0) return;
-    k_to_patch<<<grid_for(nelem), kBlock, 0, s>>>(a, b, perm, nelem);
+    k_to_patch6<<<grid_for(nelem), kBlock, 0, s>>>(dNx, dNy, dN6, perm, nelem);
     CFD_HIP(hipGetLastError());
 }
 
