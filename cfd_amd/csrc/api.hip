@@ -665,8 +665,8 @@ struct cfd_session {
     DBuf<double> LAP, LDIAG, c_calc, c_fuente, DXPOS, DYPOS, POSAUX, POSAUY, BZERO, partials;
     // fused paths: shape gradients and ESTAB outputs in patch order, binary32
     // (precision 1) or FP64 (precision 0)
-    DBuf<float> DN32, NU32, TA32, TB32, TC32, FAC32;
-    DBuf<double> DN6, DN64, NU64, TA64, TB64, TC64, FAC64;
+    DBuf<float> DN32, NT32, FAC32;
+    DBuf<double> DN6, DN64, NT64, FAC64;
     PatchSet patches;
     DBuf<int> ILAUX, ISET;
     DBuf<double> FPART;  // the forces' block partials
@@ -768,12 +768,9 @@ struct cfd_session {
 
     // the fused paths' ESTAB outputs (patch order)
     template <class R>
-    ElemP<R> elem_patch(R* nu, R* ta, R* tb, R* tc, R* fac) {
+    ElemP<R> elem_patch(R* nt, R* fac) {
         ElemP<R> o;
-        o.nu = nu;
-        o.ta = ta;
-        o.tb = tb;
-        o.tc = tc;
+        o.nt = nt;
         o.fac = fac;
         o.area = AREA.p;
         o.dtl = lazy_dt() ? nullptr : DTL.p;
@@ -781,8 +778,8 @@ struct cfd_session {
         o.cte = c.cte;
         return o;
     }
-    Elem32 elem32() { return elem_patch(NU32.p, TA32.p, TB32.p, TC32.p, FAC32.p); }
-    Elem64 elem64() { return elem_patch(NU64.p, TA64.p, TB64.p, TC64.p, FAC64.p); }
+    Elem32 elem32() { return elem_patch(NT32.p, FAC32.p); }
+    Elem64 elem64() { return elem_patch(NT64.p, FAC64.p); }
     bool fused() const { return d.precision == 0 || d.precision == 1; }
 
     // the shape gradients read by the fused kernels, in patch order
@@ -893,9 +890,9 @@ struct cfd_session {
         ElemIn in{U(), nullptr, T.p, DNX.p, DNY.p, AREA.p, SHOC.p, DTL.p, T1.p, T2.p, T3.p,
                   fu ? WX.p : nullptr, fu ? WY.p : nullptr};
         PatchIn<float> in32{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DN32.p,
-                            FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
+                            FAC32.p, NT32.p};
         PatchIn<double> in64{U(), T.p, fu ? WX.p : nullptr, fu ? WY.p : nullptr, DN64.p,
-                             FAC64.p, NU64.p, TA64.p, TB64.p, TC64.p};
+                             FAC64.p, NT64.p};
         StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p, WX.p,
                       WY.p};
         const int nrk = d.nrk > 0 ? d.nrk : 4;
@@ -1174,11 +1171,13 @@ int cfd_session_create(cfd_session** out, cfd_mesh* m, const cfd_case_desc* d) {
         S->LAP.alloc(m->t.nnz);
         if (d->precision == 1) {
             S->DN32.alloc(4 * (size_t)ne);
-            for (auto* b : {&S->NU32, &S->TA32, &S->TB32, &S->TC32, &S->FAC32}) b->alloc(ne);
+            S->NT32.alloc(4 * (size_t)ne);
+            S->FAC32.alloc(ne);
         } else if (d->precision == 0) {
             S->DN6.alloc(6 * (size_t)ne);
             S->DN64.alloc(4 * (size_t)ne);
-            for (auto* b : {&S->NU64, &S->TA64, &S->TB64, &S->TC64, &S->FAC64}) b->alloc(ne);
+            S->NT64.alloc(4 * (size_t)ne);
+            S->FAC64.alloc(ne);
         }
         if (S->fused()) {
             std::vector<int> inp(3 * (size_t)ne);

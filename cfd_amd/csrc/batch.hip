@@ -42,8 +42,8 @@ struct cfd_batch {
     int ucur = 0;
     DBuf<double> RHS, T, VX, VY, WX, WY, X, Y, RHO, E, P, RMACH, M, GAMM;
     DBuf<double> DTL, DT, AREA, DNX, DNY, HH, HHX, HHY, SHOC, T1, T2, T3;
-    DBuf<float> DN32, NU32, TA32, TB32, TC32, FAC32;
-    DBuf<double> DN6, DN64, NU64, TA64, TB64, TC64, FAC64;
+    DBuf<float> DN32, NT32, FAC32;
+    DBuf<double> DN6, DN64, NT64, FAC64;
     DBuf<int> ecase, chunk, chunk_case, pb;
     DBuf<Consts> cases;  // per case
     DBuf<double> ctl, partials, hmin;
@@ -75,12 +75,9 @@ struct cfd_batch {
     }
 
     template <class R>
-    ElemP<R> elem_patch(R* nu, R* ta, R* tb, R* tc, R* fac) {
+    ElemP<R> elem_patch(R* nt, R* fac) {
         ElemP<R> o;
-        o.nu = nu;
-        o.ta = ta;
-        o.tb = tb;
-        o.tc = tc;
+        o.nt = nt;
         o.fac = fac;
         o.area = AREA.p;
         o.dtl = DTL.p;
@@ -91,8 +88,8 @@ struct cfd_batch {
         o.dt_stride = X_NCTL;
         return o;
     }
-    Elem32 elem32() { return elem_patch(NU32.p, TA32.p, TB32.p, TC32.p, FAC32.p); }
-    Elem64 elem64() { return elem_patch(NU64.p, TA64.p, TB64.p, TC64.p, FAC64.p); }
+    Elem32 elem32() { return elem_patch(NT32.p, FAC32.p); }
+    Elem64 elem64() { return elem_patch(NT64.p, FAC64.p); }
 
     void refresh_patch_geometry() {
         if (precision == 1) {
@@ -139,9 +136,9 @@ struct cfd_batch {
             launch_estab_patch(t, patches, c, U(), T.p, GAMM.p, VX.p, VY.p, WX.p, WY.p, DNX.p,
                                DNY.p, DN6.p, dtmin, elem64(), s);
         PatchIn<float> in32{U(), T.p, nullptr, nullptr, DN32.p,
-                            FAC32.p, NU32.p, TA32.p, TB32.p, TC32.p};
+                            FAC32.p, NT32.p};
         PatchIn<double> in64{U(), T.p, nullptr, nullptr, DN64.p,
-                             FAC64.p, NU64.p, TA64.p, TB64.p, TC64.p};
+                             FAC64.p, NT64.p};
         StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p,
                       WX.p, WY.p};
         for (int irk = 1; irk <= nrk; ++irk) {
@@ -367,11 +364,13 @@ int cfd_batch_create(cfd_batch** out, int ncases, const cfd_batch_case* cases, i
         B->DNY.alloc(3 * (size_t)ne);
         if (precision == 1) {
             B->DN32.alloc(4 * (size_t)ne);
-            for (auto* b : {&B->NU32, &B->TA32, &B->TB32, &B->TC32, &B->FAC32}) b->alloc(ne);
+            B->NT32.alloc(4 * (size_t)ne);
+            B->FAC32.alloc(ne);
         } else {
             B->DN6.alloc(6 * (size_t)ne);
             B->DN64.alloc(4 * (size_t)ne);
-            for (auto* b : {&B->NU64, &B->TA64, &B->TB64, &B->TC64, &B->FAC64}) b->alloc(ne);
+            B->NT64.alloc(4 * (size_t)ne);
+            B->FAC64.alloc(ne);
         }
         for (auto* b : {&B->Ub[0], &B->Ub[1], &B->RHS, &B->P, &B->RMACH, &B->DTL, &B->DT})
             b->zero(s);

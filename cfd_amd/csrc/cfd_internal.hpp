@@ -219,14 +219,15 @@ void launch_deltat_chunks(const Tables& t, const Consts& c, const double* area, 
                           const double* VX, const double* VY, const double* WX, const double* WY,
                           double* DT, const int* chunk, int nchunks, double* partials,
                           const int* chunk_case, const Consts* cases, hipStream_t s);
-// Inputs of the patch element residual (nelem values each, patch order, of
-// the element type R: float on the FP32 path, double on the fused FP64
-// path), written by ESTAB in place of its element-order outputs: nu =
-// SHOC*CTE, tau1..3, fac = area*dtl/3; element e's values sit at patch
-// position inv_perm[e].
+// Inputs of the patch element residual (patch order, of the element type R:
+// float on the FP32 path, double on the fused FP64 path), written by ESTAB
+// in place of its element-order outputs: per element one record of nu =
+// SHOC*CTE and tau1..3, and fac = area*dtl/3; element e's values sit at
+// patch position inv_perm[e].
 template <class R>
 struct ElemP {
-    R *nu = nullptr, *ta = nullptr, *tb = nullptr, *tc = nullptr, *fac = nullptr;
+    R* nt = nullptr;   // 4 per element: nu, tau1, tau2, tau3 (16- or 32-byte records)
+    R* fac = nullptr;
     const double *area = nullptr, *dtl = nullptr;  // dtl null: DTL = DTMIN everywhere
     const int* inv_perm = nullptr;  // element e's patch position
     double cte = 1.0;
@@ -358,7 +359,8 @@ struct PatchIn {
     // and third shape functions; the first is minus their sum), 16- or
     // 32-byte aligned
     const R* dN;
-    const R *fac, *nu, *ta, *tb, *tc;
+    const R* fac;
+    const R* nt;  // per element nu, tau1, tau2, tau3 (one 16- or 32-byte record)
 };
 template <class R>
 void launch_patch_stage(const Tables& t, const PatchSet& ps, const Consts& c, const BcTable& bc,

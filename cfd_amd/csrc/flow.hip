@@ -253,18 +253,17 @@ k_estab(const int* __restrict__ inpoel, const double* __restrict__ U, const doub
 // SHOC, T_SUGN1..3 from the patch-order copies (fused paths): element e's
 // values sit at patch position inv_perm[e]
 template <class R>
-__global__ void k_estab_widen(const R* __restrict__ nu, const R* __restrict__ ta,
-                              const R* __restrict__ tb, const R* __restrict__ tc,
-                              const int* __restrict__ inv_perm, double inv_cte, int nelem,
-                              double* __restrict__ shoc, double* __restrict__ t1,
-                              double* __restrict__ t2, double* __restrict__ t3) {
+__global__ void k_estab_widen(const R* __restrict__ nt, const int* __restrict__ inv_perm,
+                              double inv_cte, int nelem, double* __restrict__ shoc,
+                              double* __restrict__ t1, double* __restrict__ t2,
+                              double* __restrict__ t3) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nelem) return;
-    const int q = inv_perm[e];
-    shoc[e] = (double)nu[q] * inv_cte;
-    t1[e] = ta[q];
-    t2[e] = tb[q];
-    t3[e] = tc[q];
+    const long long q = inv_perm[e];
+    shoc[e] = (double)nt[4 * q] * inv_cte;
+    t1[e] = nt[4 * q + 1];
+    t2[e] = nt[4 * q + 2];
+    t3[e] = nt[4 * q + 3];
 }
 
 // ---------------------------------------------------------------------------
@@ -860,7 +859,8 @@ k_patch_stage(PatchDev pd, ElemConstsP<R> kc, PatchIn<R> in, StageDev a) {
             const V4 gN = *reinterpret_cast<const V4*>(in.dN + 4 * e);
             const R Nx[3] = {-(gN.x + gN.y), gN.x, gN.y};
             const R Ny[3] = {-(gN.z + gN.w), gN.z, gN.w};
-            const R tau[3] = {in.ta[e], in.tb[e], in.tc[e]};
+            const V4 t4 = *reinterpret_cast<const V4*>(in.nt + 4 * e);  // nu, tau1..3
+            const R tau[3] = {t4.y, t4.z, t4.w};
             const double T3[3] = {sT[ln[0]], sT[ln[1]], sT[ln[2]]};
             double wx3[3] = {0.0, 0.0, 0.0}, wy3[3] = {0.0, 0.0, 0.0};
             if (kFuente) {
@@ -872,7 +872,7 @@ k_patch_stage(PatchDev pd, ElemConstsP<R> kc, PatchIn<R> in, StageDev a) {
             }
             V4 out[3];
             elem_residual<R, kFuente>(kl, sU[ln[0]], sU[ln[1]], sU[ln[2]], T3, wx3, wy3, Nx, Ny,
-                                      in.fac[e], in.nu[e], tau, out);
+                                      in.fac[e], t4.x, tau, out);
 #pragma unroll
             for (int k = 0; k < 3; ++k) sC[ls[k]] = out[k];
         }
@@ -1011,10 +1011,8 @@ k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
             double sh, rrr, s2, s3;
             estab_elem(DNX, DNY, rho, Tn, vxn, vyn, wxn, wyn, gmn, fr, tinf, rinf, dtmin, sh, rrr,
                        s2, s3);
-            o.nu[e] = (R)(sh * cte);
-            o.ta[e] = (R)rrr;
-            o.tb[e] = (R)s2;
-            o.tc[e] = (R)s3;
+            *reinterpret_cast<typename Vec4<R>::type*>(o.nt + 4 * e) =
+                make4((R)(sh * cte), (R)rrr, (R)s2, (R)s3);
             // DTL = DTMIN in every element unless local time steps blend in DT
             o.fac[e] = (R)(o.area[eo] * (o.dtl ? o.dtl[eo] : dtmin) / 3.0);
         }
@@ -1117,7 +1115,7 @@ void launch_estab_patch(const Tables& t, const PatchSet& ps, const Consts& c, co
                         const double* WX, const double* WY, const double* dNx, const double* dNy,
                         const double* dN6_patch, const double* dtmin,
                         const ElemP<R>& o, hipStream_t s) {
-    CFD_CHECK(o.nu && ps.perm.p, "internal: patch outputs missing");
+    CFD_CHECK(o.nt && ps.perm.p, "internal: patch outputs missing");
     CFD_CHECK(ps.max_nodes <= kPatchMaxNodes, "internal: patch beyond the node budget");
     PatchDev pd{};
     pd.loc = ps.loc.p;
@@ -1140,8 +1138,8 @@ void launch_estab_patch(const Tables& t, const PatchSet& ps, const Consts& c, co
 template <class R>
 void launch_estab_widen(const ElemP<R>& o, const int* inv_perm, int nelem, double* shoc,
                         double* t1, double* t2, double* t3, hipStream_t s) {
-    k_estab_widen<R><<<grid_for(nelem), kBlock, 0, s>>>(o.nu, o.ta, o.tb, o.tc, inv_perm,
-                                                        1.0 / o.cte, nelem, shoc, t1, t2, t3);
+    k_estab_widen<R><<<grid_for(nelem), kBlock, 0, s>>>(o.nt, inv_perm, 1.0 / o.cte, nelem,
+                                                        shoc, t1, t2, t3);
     CFD_HIP(hipGetLastError());
 }
 
