@@ -95,8 +95,7 @@ struct cfd_batch {
         if (precision == 1) {
             launch_to_patch_grad(DNX.p, DNY.p, DN32.p, patches.perm.p, ne, s);
         } else {
-            launch_to_patch6(DNX.p, DNY.p, DN6.p, patches.perm.p, ne, s);  // ESTAB's exact copy
-            launch_to_patch_grad(DNX.p, DNY.p, DN64.p, patches.perm.p, ne, s);
+            launch_to_patch64(DNX.p, DNY.p, DN6.p, DN64.p, patches.perm.p, ne, s);
         }
     }
 

@@ -374,10 +374,11 @@ void patch_make_persistent(PatchSet& ps, bool fuente);
 // slot ranges) fits its 64 KB budget for patches of pe elements and up to
 // max_nodes nodes
 bool patch_lds_fits(int max_nodes, int pe, size_t slot_bytes);
-// dNx/dNy in patch order, exact, 6 per element (Nx1..3 then Ny1..3: ESTAB's
-// inputs on the fused FP64 path)
-void launch_to_patch6(const double* dNx, const double* dNy, double* dN6, const int* perm,
-                      int nelem, hipStream_t s);
+// dNx/dNy in patch order for the fused FP64 path, from one gather: dN6 holds
+// 6 per element (Nx1..3 then Ny1..3, ESTAB's inputs), dN4 the stage kernel's
+// 4 (Nx2, Nx3, Ny2, Ny3); both exact
+void launch_to_patch64(const double* dNx, const double* dNy, double* dN6, double* dN4,
+                       const int* perm, int nelem, hipStream_t s);
 // The fused kernel's gradients in patch order, rounded to R: 4 per element
 // (Nx2, Nx3, Ny2, Ny3 of the element at each patch position)
 template <class R>

@@ -1020,17 +1020,22 @@ k_estab_patch(PatchDev pd, EstabIn ei, ElemP<R> o) {
     }
 }
 
-// b[6q+k] = nx[3 perm[q] + k], b[6q+3+k] = ny[3 perm[q] + k]
-__global__ void k_to_patch6(const double* __restrict__ nx, const double* __restrict__ ny,
-                            double* __restrict__ b, const int* __restrict__ perm, int nelem) {
+// One gather of the element's six gradients, two outputs in patch order:
+// b6[6q+k] = nx[3 perm[q] + k], b6[6q+3+k] = ny[3 perm[q] + k] (ESTAB's record,
+// three 16-byte stores) and b4[4q..4q+3] = Nx2, Nx3, Ny2, Ny3 (the stage kernel's)
+__global__ void k_to_patch64(const double* __restrict__ nx, const double* __restrict__ ny,
+                             double* __restrict__ b6, double* __restrict__ b4,
+                             const int* __restrict__ perm, int nelem) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nelem) return;
     const long long e = perm[q];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        b[6 * (long long)q + k] = nx[3 * e + k];
-        b[6 * (long long)q + 3 + k] = ny[3 * e + k];
-    }
+    const double x0 = nx[3 * e], x1 = nx[3 * e + 1], x2 = nx[3 * e + 2];
+    const double y0 = ny[3 * e], y1 = ny[3 * e + 1], y2 = ny[3 * e + 2];
+    double2* o6 = reinterpret_cast<double2*>(b6 + 6 * (long long)q);
+    o6[0] = make_double2(x0, x1);
+    o6[1] = make_double2(x2, y0);
+    o6[2] = make_double2(y1, y2);
+    *reinterpret_cast<typename Vec4<double>::type*>(b4 + 4 * (long long)q) = make4(x1, x2, y1, y2);
 }
 
 template <class R>
@@ -1327,10 +1332,10 @@ void patch_make_persistent(PatchSet& ps, bool fuente) {
 template void patch_make_persistent<float>(PatchSet&, bool);
 template void patch_make_persistent<double>(PatchSet&, bool);
 
-void launch_to_patch6(const double* dNx, const double* dNy, double* dN6, const int* perm,
-                      int nelem, hipStream_t s) {
+void launch_to_patch64(const double* dNx, const double* dNy, double* dN6, double* dN4,
+                       const int* perm, int nelem, hipStream_t s) {
     if (nelem <= 0) return;
-    k_to_patch6<<<grid_for(nelem), kBlock, 0, s>>>(dNx, dNy, dN6, perm, nelem);
+    k_to_patch64<<<grid_for(nelem), kBlock, 0, s>>>(dNx, dNy, dN6, dN4, perm, nelem);
     CFD_HIP(hipGetLastError());
 }
 
@@ -1350,7 +1355,5 @@ void launch_to_patch_grad(const double* dNx, const double* dNy, R* dN, const int
 }
 template void launch_to_patch_grad<float>(const double*, const double*, float*, const int*, int,
                                           hipStream_t);
-template void launch_to_patch_grad<double>(const double*, const double*, double*, const int*, int,
-                                           hipStream_t);
 
 }  // namespace cfd
