@@ -13,8 +13,10 @@ value = 4 * nelem * steps / timed seconds (every calcRHS evaluation complete,
 none hoisted).
 
 Every path is timed on a developed state: the FP64 path first runs the
-configuration's 500 steps from the free stream (untimed), and its state is
-loaded into each timed session.  The timed region starts with all inputs
+configuration's 500 steps from the free stream (untimed) and goes on in the
+same session with its warm-up and timed steps; the state after those is
+loaded into each other timed session and is the accuracy section's FP64
+reference.  The timed region starts with all inputs
 resident in HBM and is bracketed by a stream synchronisation on both sides.
 Per-kernel times come from HIP events recorded on the session's own stream
 inside the timed region.
@@ -22,7 +24,7 @@ inside the timed region.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32|fp64ref]
 
 Beside the headline line (all in the same JSON object): "accuracy" (the FP32
-element path against the FP64 path after the 500 steps: L2 density error,
+element path against the FP64 path after the same steps: L2 density error,
 drag and lift coefficients, with the round-off chaos floor), "ale" (the
 pitching ellipse 512 x 1024, learned warm start + FP32 elements against the
 plain FP64 path at equal PCG tolerance), "batch" (256 uneven cases),
@@ -444,13 +446,20 @@ def develop(cs, nsteps: int, prec: int = 0, renumber: bool = True, ulp_node=None
         S.set("U", U)
         S.set("U1", U)
     S.step(nsteps)
+    st, sc, finite = session_state(S)
+    S.close()
+    return st, sc, finite
+
+
+def session_state(S):
+    """The session's state (the fields the next step reads and the step
+    scalars), its scalars, and whether the state is finite."""
+    import numpy as np
     S.sync()
     sc = S.scalars()
     st = {k: S.get(k) for k in DEVELOP_FIELDS}
     st.update({k: sc[k] for k in ("DTMIN", "TIME", "DTMIN1", "BANDERA", "ITER")})
-    finite = bool(np.isfinite(st["U"]).all())
-    S.close()
-    return st, sc, finite
+    return st, sc, bool(np.isfinite(st["U"]).all())
 
 
 def accuracy(cs, dev: dict, dev_sc: dict, nsteps: int) -> dict:
@@ -481,15 +490,27 @@ def accuracy(cs, dev: dict, dev_sc: dict, nsteps: int) -> dict:
     return out
 
 
-def run_path(cs, prec: int, steps: int, warmup: int, start: dict, breakdown: bool = False,
-             renumber: bool = True) -> dict:
-    """Loads the developed state `start`, runs `warmup` untimed steps, then
+def run_path(cs, prec: int, steps: int, warmup: int, start: dict | None,
+             breakdown: bool = False, renumber: bool = True, session=None) -> dict:
+    """Loads the developed state `start` (or continues `session`, the open
+    session that is developing it, and leaves it open), runs `warmup` untimed
+    steps, then
     times `steps` whole steps; HIP events on the session's stream time the
-    element kernel launches inside the timed region.  The fused paths run
+    element kernel launches inside the timed region (on the fused paths one
+    event pair around each step's four back-to-back stage launches, on the
+    reference-order path one pair per launch; the events are created before
+    the timed region).  The fused paths run
     with the session's patch-order numbering (renumber = 1: the same results
     bit for bit on a fixed mesh)."""
-    S = _session(cs, prec, renumber)
-    S.set_state(start, fields=DEVELOP_FIELDS)
+    if session is None:
+        S = _session(cs, prec, renumber)
+        S.set_state(start, fields=DEVELOP_FIELDS)
+    else:
+        S = session
+    # timing on before the warm-up, so that the session's event pool is
+    # built before it and the timed steps follow the warm-up without an idle
+    # gap; the second call only resets the accumulators
+    S.timing(True, slots=["ELEMENT"])
     S.step(warmup)
     S.sync()
     S.timing(True, slots=["ELEMENT"])
@@ -517,7 +538,8 @@ def run_path(cs, prec: int, steps: int, warmup: int, start: dict, breakdown: boo
                 kern[k.lower()] = {"avg_us": round(t / n * 1e3, 2), "launches": n}
         S.timing(False)
         r["kernels"] = kern
-    S.close()
+    if session is None:
+        S.close()
     return r
 
 
@@ -567,11 +589,26 @@ def main():
     cs = cylinder_bench_case(args.nr, args.nt)
     prec = PRECISION[args.precision]
     rn = not args.no_renumber
-    dev, dev_sc, dev_finite = develop(cs, args.develop, prec=0, renumber=rn)
+    if prec == 0:
+        # the FP64 path's warm-up and timed steps follow its development in
+        # the same session, with no idle gap between them (the GPU's clocks
+        # fall while it waits); the state that the accuracy section and the
+        # other paths start from is taken after the timed and breakdown steps
+        dev_s = _session(cs, 0, rn)
+        dev_s.step(args.develop)
+        r = run_path(cs, 0, args.steps, args.warmup, None, breakdown=True, renumber=rn,
+                     session=dev_s)
+        dev, dev_sc, dev_finite = session_state(dev_s)
+        dev_s.close()
+        dev_steps = args.develop + args.warmup + args.steps + min(args.steps, 10)
+    else:
+        dev, dev_sc, dev_finite = develop(cs, args.develop, prec=0, renumber=rn)
+        dev_steps = args.develop
     if not dev_finite:
         raise RuntimeError("the developed state is not finite")
-    progress(f"developed state: {args.develop} FP64 steps, t = {dev['TIME']:.4g}")
-    r = run_path(cs, prec, args.steps, args.warmup, dev, breakdown=True, renumber=rn)
+    progress(f"developed state: {dev_steps} FP64 steps, t = {dev['TIME']:.4g}")
+    if prec != 0:
+        r = run_path(cs, prec, args.steps, args.warmup, dev, breakdown=True, renumber=rn)
     DTYPE = {0: "f64", 1: "f32-element/f64-accumulate", 2: "f64"}
     PATH = {0: "fused FP64", 1: "fused FP32-element", 2: "FP64 reference-order"}
     out = {
@@ -614,8 +651,8 @@ def main():
         "state": r["state"],
     }
     progress(f"main path: {r['ms_per_step']:.3f} ms/step")
-    out["accuracy"] = accuracy(cs, dev, dev_sc, args.develop)
-    progress(f"accuracy: FP32 L2(rho) {out['accuracy']['fp32_l2_rho']:.3e} after {args.develop} steps")
+    out["accuracy"] = accuracy(cs, dev, dev_sc, dev_steps)
+    progress(f"accuracy: FP32 L2(rho) {out['accuracy']['fp32_l2_rho']:.3e} after {dev_steps} steps")
     if not args.no_variants:
         # the other two element paths, shorter runs of the same workload
         out["variants"] = {}

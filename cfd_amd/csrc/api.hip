@@ -701,7 +701,7 @@ struct cfd_session {
     long long tcnt[CFD_NKTIMERS] = {0};
     std::vector<hipEvent_t> pool;
     hipEvent_t open_ev[CFD_NKTIMERS] = {};
-    struct Pend { hipEvent_t a, b; int slot; };
+    struct Pend { hipEvent_t a, b; int slot, launches; };
     std::vector<Pend> pending;
 
     double* U() { return Ub[ucur].p; }
@@ -721,11 +721,12 @@ struct cfd_session {
         open_ev[slot] = ev();
         CFD_HIP(hipEventRecord(open_ev[slot], s));
     }
-    void te(int slot) {
+    // closes the slot's interval, which held `launches` back-to-back launches
+    void te(int slot, int launches = 1) {
         if (!timing || !((tmask >> slot) & 1u)) return;
         hipEvent_t b = ev();
         CFD_HIP(hipEventRecord(b, s));
-        pending.push_back({open_ev[slot], b, slot});
+        pending.push_back({open_ev[slot], b, slot, launches});
     }
     void resolve() {
         for (auto& p : pending) {
@@ -733,7 +734,7 @@ struct cfd_session {
             float ms = 0.f;
             CFD_HIP(hipEventElapsedTime(&ms, p.a, p.b));
             tacc[p.slot] += ms;
-            tcnt[p.slot] += 1;
+            tcnt[p.slot] += p.launches;
             pool.push_back(p.a);
             pool.push_back(p.b);
         }
@@ -895,15 +896,17 @@ struct cfd_session {
         StageNodes sn{U(), U1(), RHS.p, T.p, VX.p, VY.p, RHO.p, E.p, P.p, RMACH.p, M.p, GAMM.p, WX.p,
                       WY.p};
         const int nrk = d.nrk > 0 ? d.nrk : 4;
+        // fused paths: residual + gather + stage in one launch per stage, the
+        // stages back to back, timed as the element kernel by one event pair
+        // around all of them (an event record costs the stream a few us)
+        if (fused()) tb(CFD_K_ELEMENT);
         for (int irk = 1; irk <= nrk; ++irk) {
             double rk_fact = 1.0 / (nrk + 1 - irk);
-            if (fused()) {  // residual + gather + stage fused (timed as the element kernel)
-                tb(CFD_K_ELEMENT);
+            if (fused()) {
                 if (d.precision == 1)
                     launch_patch_stage(t, patches, c, bc.tab, rk_fact, in32, sn, irk == nrk, s);
                 else
                     launch_patch_stage(t, patches, c, bc.tab, rk_fact, in64, sn, irk == nrk, s);
-                te(CFD_K_ELEMENT);
                 continue;
             }
             tb(CFD_K_ELEMENT);
@@ -914,6 +917,7 @@ struct cfd_session {
                                 irk == nrk, s);
             te(CFD_K_STAGE);
         }
+        if (fused()) te(CFD_K_ELEMENT, nrk);
 
         // fluidStructure (meshMove.f90:52-127)
         if (d.gcl >= 0) {
@@ -1391,6 +1395,19 @@ int cfd_session_timing_mask(cfd_session* S, unsigned mask) {
         S->resolve();
         S->timing = mask != 0;
         S->tmask = mask;
+        // the events come from the pool: created here and recorded once,
+        // before the timed steps (an event's first record costs the stream
+        // tens of us, a later one a few)
+        constexpr size_t kEventPool = 256;
+        bool fresh = false;
+        while (S->timing && S->pool.size() < kEventPool) {
+            hipEvent_t e;
+            CFD_HIP(hipEventCreate(&e));
+            CFD_HIP(hipEventRecord(e, S->s));
+            S->pool.push_back(e);
+            fresh = true;
+        }
+        if (fresh) CFD_HIP(hipStreamSynchronize(S->s));
         for (int k = 0; k < CFD_NKTIMERS; ++k) {
             S->tacc[k] = 0.0;
             S->tcnt[k] = 0;

@@ -565,6 +565,42 @@ def test_session_free_run_and_determinism(cyl):
     assert np.all(rel < 5e-2), rel
 
 
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_session_timing_counts_and_leaves_results(cyl, precision):
+    """HIP-event timing of the session's slots: the element slot counts every
+    stage launch (4 a step; on the fused paths one event pair spans the
+    step's stages), one interval a step for deltat, ESTAB and the whole step,
+    positive totals, and the state after timed steps equal bit for bit to
+    the untimed run's."""
+    cs, _ = cyl
+    from cfd_amd.solver import Session
+    runs = []
+    for timed in (False, True):
+        S = Session(cs, precision=precision)
+        if timed:
+            S.timing(True, slots=["ELEMENT"])
+            S.step(3)
+            S.sync()
+            t, n = S.kernel_time("ELEMENT")
+            assert n == 12 and t > 0.0, (t, n)
+            S.timing(True)  # every slot
+            S.step(2)
+            S.sync()
+            counts = {"ELEMENT": 8, "DELTAT": 2, "ESTAB": 2, "STEP": 2}
+            if precision == 2:
+                counts["STAGE"] = 8
+            for k, nk in counts.items():
+                t, n = S.kernel_time(k)
+                assert n == nk and t > 0.0, (k, t, n)
+            S.timing(False)
+        else:
+            S.step(5)
+        runs.append(S.state())
+        S.close()
+    for k in ("U", "T", "RHS", "P"):
+        assert bitdiff(runs[0][k], runs[1][k]) == 0, k
+
+
 def test_estab_bitwise_fraction(cyl, mctx):
     """Diagnostic: share of ESTAB outputs identical to the last bit (only the
     pow() calls can differ)."""
