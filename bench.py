@@ -591,9 +591,10 @@ def main():
     rn = not args.no_renumber
     if prec == 0:
         # the FP64 path's warm-up and timed steps follow its development in
-        # the same session, with no idle gap between them (the GPU's clocks
-        # fall while it waits); the state that the accuracy section and the
-        # other paths start from is taken after the timed and breakdown steps
+        # the same session, with no idle gap between them (after an idle gap
+        # the GPU runs the next few dozen steps slower); the state that the
+        # accuracy section and the other paths start from is taken after the
+        # timed and breakdown steps
         dev_s = _session(cs, 0, rn)
         dev_s.step(args.develop)
         r = run_path(cs, 0, args.steps, args.warmup, None, breakdown=True, renumber=rn,

@@ -1396,8 +1396,8 @@ int cfd_session_timing_mask(cfd_session* S, unsigned mask) {
         S->timing = mask != 0;
         S->tmask = mask;
         // the events come from the pool: created here and recorded once,
-        // before the timed steps (an event's first record costs the stream
-        // tens of us, a later one a few)
+        // so that no event is created or used for the first time while the
+        // timed steps are queued
         constexpr size_t kEventPool = 256;
         bool fresh = false;
         while (S->timing && S->pool.size() < kEventPool) {
