@@ -309,7 +309,9 @@ enum {
 };
 /* kernel timing slots (HIP events on the session stream) */
 enum {
-    CFD_K_ELEMENT = 0,  /* calcRHS+FUENTE element kernel                      */
+    CFD_K_ELEMENT = 0,  /* calcRHS+FUENTE element kernel (fused paths: the
+                           fused stage launches, one event pair around each
+                           step's back-to-back stages)                       */
     CFD_K_STAGE,        /* gather + stage update                              */
     CFD_K_ESTAB, CFD_K_DELTAT, CFD_K_PCG, CFD_K_GEOM, CFD_K_STEP, CFD_NKTIMERS
 };
@@ -353,10 +355,14 @@ int cfd_session_read_restart(cfd_session *s, const char *path, int keep_clock);
 /* the CFD_S_* scalars; get waits for queued steps */
 int cfd_session_get_scalars(cfd_session *s, double *scalars /* CFD_NSCALARS */);
 int cfd_session_set_scalars(cfd_session *s, const double *scalars);
-/* per-slot kernel time: enable (all slots) resets the accumulators */
+/* per-slot kernel time: enable (all slots) resets the accumulators and
+ * creates the session's events (a pool of 256, each recorded once) before
+ * any timed step */
 int cfd_session_timing(cfd_session *s, int enable);
 /* the same for the slots whose bit (1 << CFD_K_*) is set in mask */
 int cfd_session_timing_mask(cfd_session *s, unsigned mask);
+/* total_ms over the slot's timed intervals since enable; launches = the
+ * kernel launches those intervals hold (total_ms / launches = mean launch) */
 int cfd_session_kernel_time(cfd_session *s, int slot, double *total_ms, long long *launches);
 
 /* ---- batches of independent fixed-mesh cases ----------------------------
